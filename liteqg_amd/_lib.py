@@ -1,0 +1,77 @@
+"""ctypes binding of liteqg_amd/liblqg.so (the C ABI in include/liteqg.h).
+
+The library is the only compute path: if it is missing this module raises,
+there is no fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblqg.so")
+
+LQG_OK = 0
+ORDER = {"reference": 0, "nearest": 1}
+
+
+class LqgError(RuntimeError):
+    """A failure reported by the C ABI (status code + message)."""
+
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"[lqg status {status}] {msg}")
+        self.status = status
+
+
+class Stats(C.Structure):
+    _fields_ = [("queries", C.c_uint64), ("expansions", C.c_uint64),
+                ("evaluations", C.c_uint64), ("device_ms", C.c_double)]
+
+
+_lib = None
+
+# name -> (restype, argtypes); every function returns lqg_status (int)
+_P = C.c_void_p
+_SIGS = {
+    "lqg_create": [C.c_char_p, C.c_char_p, C.c_uint64, C.c_uint64, C.c_uint64, C.POINTER(_P)],
+    "lqg_destroy": [_P],
+    "lqg_load": [_P, C.c_char_p],
+    "lqg_save": [_P, C.c_char_p],
+    "lqg_set_ef": [_P, C.c_uint32],
+    "lqg_set_order": [_P, C.c_int],
+    "lqg_build_index": [_P, _P, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32],
+    "lqg_search": [_P, _P, C.c_uint32, _P],
+    "lqg_search_batch": [_P, _P, C.c_uint64, _P, _P, C.c_uint32, C.c_int, _P, _P, _P],
+    "lqg_search_batch_device": [_P, _P, C.c_uint64, _P, _P, C.c_uint32, C.c_uint32, C.c_int, _P,
+                                _P, _P, _P],
+    "lqg_info": [_P, _P, _P, _P, _P, _P],
+    "lqg_get_graph": [_P, _P],
+}
+EXPORTED = tuple(_SIGS) + ("lqg_last_error",)
+
+
+def lib():
+    """Load liblqg.so once; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m liteqg_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        L.lqg_last_error.argtypes = []
+        L.lqg_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != LQG_OK:
+        msg = lib().lqg_last_error()
+        raise LqgError(status, msg.decode() if msg else "")
+
+
+def ptr(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)

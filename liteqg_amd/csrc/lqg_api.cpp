@@ -1,0 +1,471 @@
+// lqg_api.cpp -- host side of the C ABI declared in include/liteqg.h.
+//
+// Owns the index (host copy for save(), device copy for search), validates
+// every call the way the reference's binding should have (python/bindings.cpp),
+// and launches the gfx950 kernels through lqg_internal.h.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/liteqg.h"
+#include "lqg_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+lqg_status fail(lqg_status s, const std::string& msg) {
+    g_err = msg;
+    return s;
+}
+
+lqg_status hip_fail(hipError_t e, const char* what) {
+    return fail(LQG_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                   \
+    do {                                                \
+        hipError_t e_ = (expr);                         \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;  // elements
+    hipError_t reserve(size_t n) {
+        if (n <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) cap = n;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+int ef_class(uint32_t ef) {
+    const int classes[] = {64, 128, 256, 512, 1024};
+    for (int c : classes)
+        if (ef <= static_cast<uint32_t>(c)) return c;
+    return 0;
+}
+
+}  // namespace
+
+struct lqg_index {
+    uint64_t n = 0, dim = 0, deg = 0;
+    uint32_t ep = 0;
+    uint32_t ef = 0;
+    bool ef_set = false;
+    bool have_data = false;  // loaded or built
+    lqg_order order = LQG_ORDER_REFERENCE;
+
+    // host copy: exactly the file's content, split by section
+    std::vector<float> raw;       // n x dim
+    std::vector<uint32_t> nbr;    // n x deg
+    std::vector<uint8_t> qdata;   // n x dim
+    std::vector<float> sq8;       // dim x {min, scale}
+
+    // device copy
+    bool on_device = false;
+    hipStream_t stream = nullptr;
+    DevBuf<uint8_t> d_qdata;
+    DevBuf<uint32_t> d_nbr;
+    DevBuf<float> d_min, d_scale;
+    // per-call scratch (grow only)
+    DevBuf<float> d_q;
+    DevBuf<uint32_t> d_ids, d_ef, d_k, d_stats, d_counter;
+    DevBuf<float> d_dist;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    lqg::DevIndex dev() const {
+        lqg::DevIndex ix;
+        ix.qdata = d_qdata.p;
+        ix.nbr = d_nbr.p;
+        ix.sq8_min = d_min.p;
+        ix.sq8_scale = d_scale.p;
+        ix.n = static_cast<uint32_t>(n);
+        ix.dim = static_cast<uint32_t>(dim);
+        ix.deg = static_cast<uint32_t>(deg);
+        ix.ep = ep;
+        return ix;
+    }
+};
+
+namespace {
+
+lqg_status ensure_device(lqg_index* h) {
+    if (h->on_device) return LQG_OK;
+    if (!h->have_data) return fail(LQG_ERR_STATE, "index has no data: call load() or build_index() first");
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return fail(LQG_ERR_DEVICE, "no HIP device available");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(LQG_ERR_DEVICE, std::string("device is ") + prop.gcnArchName +
+                                        ", this library is built for gfx950 only");
+    if (!h->stream) HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    if (!h->ev0) HIP_TRY(hipEventCreate(&h->ev0));
+    if (!h->ev1) HIP_TRY(hipEventCreate(&h->ev1));
+    const size_t nd = h->n * h->dim, ng = h->n * h->deg;
+    HIP_TRY(h->d_qdata.reserve(nd));
+    HIP_TRY(h->d_nbr.reserve(ng));
+    HIP_TRY(h->d_min.reserve(h->dim));
+    HIP_TRY(h->d_scale.reserve(h->dim));
+    std::vector<float> mn(h->dim), sc(h->dim);
+    for (uint64_t i = 0; i < h->dim; ++i) {
+        mn[i] = h->sq8[2 * i];
+        sc[i] = h->sq8[2 * i + 1];
+    }
+    HIP_TRY(hipMemcpy(h->d_qdata.p, h->qdata.data(), nd, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->d_nbr.p, h->nbr.data(), ng * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->d_min.p, mn.data(), h->dim * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->d_scale.p, sc.data(), h->dim * 4, hipMemcpyHostToDevice));
+    HIP_TRY(h->d_counter.reserve(1));
+    h->on_device = true;
+    return LQG_OK;
+}
+
+lqg::LaunchCfg default_cfg(uint32_t ef_max, uint64_t nq) {
+    lqg::LaunchCfg c;
+    c.ef_cap = ef_class(ef_max);
+    c.waves_per_group = nq >= 4 ? 4 : 1;
+    c.cache_bits = 10;
+    c.groups = 0;
+    return c;
+}
+
+lqg_status check_queries(const float* q, uint64_t count) {
+    for (uint64_t i = 0; i < count; ++i)
+        if (!std::isfinite(q[i])) return fail(LQG_ERR_ARG, "query contains a non-finite value");
+    return LQG_OK;
+}
+
+lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
+                     const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, lqg_order order,
+                     uint32_t* d_results, float* d_dists, uint32_t* d_stats, hipStream_t s) {
+    const int cls = ef_class(ef_max);
+    if (cls == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported on the device");
+    lqg::SearchJob job;
+    job.queries = d_queries;
+    job.ef = d_ef;
+    job.k = d_k;
+    job.nq = static_cast<uint32_t>(nq);
+    job.ef_all = ef_max;
+    job.k_all = kmax;
+    job.kstride = kmax;
+    job.order = order == LQG_ORDER_NEAREST ? 1 : 0;
+    job.out_ids = d_results;
+    job.out_d = d_dists;
+    job.out_stats = d_stats;
+    job.counter = h->d_counter.p;
+    const lqg::LaunchCfg cfg = default_cfg(ef_max, nq);
+    HIP_TRY(lqg::launch_reset_counter(h->d_counter.p, s));
+    HIP_TRY(lqg::launch_search(h->dev(), job, cfg, s));
+    return LQG_OK;
+}
+
+bool valid_order(lqg_order o) { return o == LQG_ORDER_REFERENCE || o == LQG_ORDER_NEAREST; }
+
+}  // namespace
+
+extern "C" {
+
+const char* lqg_last_error(void) { return g_err.c_str(); }
+
+lqg_status lqg_create(const char* index_type, const char* metric, uint64_t num_elements,
+                      uint64_t dimension, uint64_t degree_bound, lqg_index** out) {
+    if (!out) return fail(LQG_ERR_ARG, "out is null");
+    *out = nullptr;
+    if (!metric || std::string(metric) != "L2")
+        return fail(LQG_ERR_ARG, "Only L2 distance supported currently");
+    if (degree_bound < 32 || degree_bound % 32 != 0)
+        return fail(LQG_ERR_ARG, "The degree bound must be a multiple of 32");
+    if (!index_type || std::string(index_type) != "QG")
+        return fail(LQG_ERR_ARG, std::string("Index type [") + (index_type ? index_type : "") +
+                                     "] not supported");
+    if (num_elements == 0 || num_elements >= 0x7FFFFFFFull)
+        return fail(LQG_ERR_ARG, "num_elements must be in [1, 2^31-1)");
+    if (dimension == 0 || dimension > 4096)
+        return fail(LQG_ERR_ARG, "dimension must be in [1, 4096]");
+    lqg_index* h = new (std::nothrow) lqg_index();
+    if (!h) return fail(LQG_ERR_NOMEM, "allocation failed");
+    h->n = num_elements;
+    h->dim = dimension;
+    h->deg = degree_bound;
+    *out = h;
+    return LQG_OK;
+}
+
+lqg_status lqg_destroy(lqg_index* h) {
+    if (!h) return LQG_OK;
+    h->d_qdata.release();
+    h->d_nbr.release();
+    h->d_min.release();
+    h->d_scale.release();
+    h->d_q.release();
+    h->d_ids.release();
+    h->d_ef.release();
+    h->d_k.release();
+    h->d_stats.release();
+    h->d_counter.release();
+    h->d_dist.release();
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return LQG_OK;
+}
+
+lqg_status lqg_load(lqg_index* h, const char* filename) {
+    if (!h || !filename) return fail(LQG_ERR_ARG, "null argument");
+    FILE* f = fopen(filename, "rb");
+    if (!f) return fail(LQG_ERR_IO, std::string("cannot open ") + filename);
+    const uint64_t n = h->n, d = h->dim, g = h->deg;
+    const uint64_t expect = 4 + n * (d + g) * 4 + n * d + 8 * d;
+    fseek(f, 0, SEEK_END);
+    const long long got = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    if (got < 0 || static_cast<uint64_t>(got) != expect) {
+        fclose(f);
+        char msg[256];
+        snprintf(msg, sizeof msg, "%s is %lld bytes, expected %llu for N=%llu D=%llu deg=%llu",
+                 filename, got, static_cast<unsigned long long>(expect),
+                 static_cast<unsigned long long>(n), static_cast<unsigned long long>(d),
+                 static_cast<unsigned long long>(g));
+        return fail(LQG_ERR_IO, msg);
+    }
+    uint32_t ep = 0;
+    std::vector<float> raw, sq8;
+    std::vector<uint32_t> nbr;
+    std::vector<uint8_t> qdata;
+    try {
+        raw.resize(n * d);
+        nbr.resize(n * g);
+        qdata.resize(n * d);
+        sq8.resize(2 * d);
+    } catch (...) {
+        fclose(f);
+        return fail(LQG_ERR_NOMEM, "host allocation failed");
+    }
+    bool ok = fread(&ep, 4, 1, f) == 1;
+    // rows: D floats then deg ids (qg.hpp:420-424), read in blocks
+    const uint64_t row = d + g;
+    const uint64_t block = std::max<uint64_t>(1, (64ull << 20) / (row * 4));
+    std::vector<uint32_t> buf;
+    try {
+        buf.resize(std::min(n, block) * row);
+    } catch (...) {
+        fclose(f);
+        return fail(LQG_ERR_NOMEM, "host allocation failed");
+    }
+    for (uint64_t s = 0; ok && s < n; s += block) {
+        const uint64_t e = std::min(n, s + block);
+        ok = fread(buf.data(), 4, (e - s) * row, f) == (e - s) * row;
+        for (uint64_t i = s; ok && i < e; ++i) {
+            memcpy(&raw[i * d], &buf[(i - s) * row], d * 4);
+            memcpy(&nbr[i * g], &buf[(i - s) * row + d], g * 4);
+        }
+    }
+    ok = ok && fread(qdata.data(), 1, n * d, f) == n * d;
+    ok = ok && fread(sq8.data(), 4, 2 * d, f) == 2 * d;
+    fclose(f);
+    if (!ok) return fail(LQG_ERR_IO, std::string("short read from ") + filename);
+    if (ep >= n) return fail(LQG_ERR_IO, "entry point is out of range: file does not match N");
+    for (uint64_t i = 0; i < n * g; ++i)
+        if (nbr[i] >= n) return fail(LQG_ERR_IO, "neighbour id out of range: file does not match N/D/deg");
+    h->ep = ep;
+    h->raw.swap(raw);
+    h->nbr.swap(nbr);
+    h->qdata.swap(qdata);
+    h->sq8.swap(sq8);
+    h->have_data = true;
+    h->on_device = false;
+    return LQG_OK;
+}
+
+lqg_status lqg_save(lqg_index* h, const char* filename) {
+    if (!h || !filename) return fail(LQG_ERR_ARG, "null argument");
+    if (!h->have_data) return fail(LQG_ERR_STATE, "index has no data to save");
+    FILE* f = fopen(filename, "wb");
+    if (!f) return fail(LQG_ERR_IO, std::string("cannot create ") + filename);
+    const uint64_t n = h->n, d = h->dim, g = h->deg, row = d + g;
+    bool ok = fwrite(&h->ep, 4, 1, f) == 1;
+    std::vector<uint32_t> r(row);
+    for (uint64_t i = 0; ok && i < n; ++i) {
+        memcpy(r.data(), &h->raw[i * d], d * 4);
+        memcpy(r.data() + d, &h->nbr[i * g], g * 4);
+        ok = fwrite(r.data(), 4, row, f) == row;
+    }
+    ok = ok && fwrite(h->qdata.data(), 1, n * d, f) == n * d;
+    ok = ok && fwrite(h->sq8.data(), 4, 2 * d, f) == 2 * d;
+    ok = (fclose(f) == 0) && ok;
+    return ok ? LQG_OK : fail(LQG_ERR_IO, std::string("write failed: ") + filename);
+}
+
+lqg_status lqg_set_ef(lqg_index* h, uint32_t ef) {
+    if (!h) return fail(LQG_ERR_ARG, "null index");
+    if (ef == 0) return fail(LQG_ERR_ARG, "EF must be positive");
+    if (ef_class(ef) == 0) return fail(LQG_ERR_UNSUPPORTED, "EF above 1024 is not supported");
+    h->ef = ef;
+    h->ef_set = true;
+    return LQG_OK;
+}
+
+lqg_status lqg_set_order(lqg_index* h, lqg_order order) {
+    if (!h) return fail(LQG_ERR_ARG, "null index");
+    if (!valid_order(order)) return fail(LQG_ERR_ARG, "unknown order");
+    h->order = order;
+    return LQG_OK;
+}
+
+lqg_status lqg_search(lqg_index* h, const float* query, uint32_t k, uint32_t* results) {
+    if (!h || !query || !results) return fail(LQG_ERR_ARG, "null argument");
+    if (!h->ef_set) return fail(LQG_ERR_STATE, "set_ef() must be called before search()");
+    if (k == 0) return fail(LQG_ERR_ARG, "k must be positive");
+    lqg_status st = check_queries(query, h->dim);
+    if (st != LQG_OK) return st;
+    st = ensure_device(h);
+    if (st != LQG_OK) return st;
+    HIP_TRY(h->d_q.reserve(h->dim));
+    HIP_TRY(h->d_ids.reserve(k));
+    HIP_TRY(hipMemcpyAsync(h->d_q.p, query, h->dim * 4, hipMemcpyHostToDevice, h->stream));
+    st = run_batch(h, h->d_q.p, 1, nullptr, nullptr, h->ef, k, h->order, h->d_ids.p, nullptr,
+                   nullptr, h->stream);
+    if (st != LQG_OK) return st;
+    HIP_TRY(hipMemcpyAsync(results, h->d_ids.p, k * 4ull, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return LQG_OK;
+}
+
+lqg_status lqg_search_batch(lqg_index* h, const float* queries, uint64_t nq, const uint32_t* ef,
+                            const uint32_t* k, uint32_t kmax, lqg_order order, uint32_t* results,
+                            float* dists, lqg_stats* stats) {
+    if (!h || !queries || !results) return fail(LQG_ERR_ARG, "null argument");
+    if (nq == 0 || nq > 0xFFFFFFFFull) return fail(LQG_ERR_ARG, "nq must be in [1, 2^32)");
+    if (kmax == 0) return fail(LQG_ERR_ARG, "kmax must be positive");
+    if (!valid_order(order)) return fail(LQG_ERR_ARG, "unknown order");
+    if (!ef && !h->ef_set) return fail(LQG_ERR_STATE, "set_ef() or per-query ef required");
+    uint32_t ef_max = ef ? 0 : h->ef;
+    if (ef) {
+        for (uint64_t i = 0; i < nq; ++i) {
+            if (ef[i] == 0) return fail(LQG_ERR_ARG, "per-query ef must be positive");
+            ef_max = std::max(ef_max, ef[i]);
+        }
+    }
+    if (k) {
+        for (uint64_t i = 0; i < nq; ++i)
+            if (k[i] == 0 || k[i] > kmax) return fail(LQG_ERR_ARG, "per-query k must be in [1, kmax]");
+    }
+    if (ef_class(ef_max) == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported");
+    lqg_status st = check_queries(queries, nq * h->dim);
+    if (st != LQG_OK) return st;
+    st = ensure_device(h);
+    if (st != LQG_OK) return st;
+    HIP_TRY(h->d_q.reserve(nq * h->dim));
+    HIP_TRY(h->d_ids.reserve(nq * kmax));
+    if (dists) HIP_TRY(h->d_dist.reserve(nq * kmax));
+    if (ef) HIP_TRY(h->d_ef.reserve(nq));
+    if (k) HIP_TRY(h->d_k.reserve(nq));
+    if (stats) HIP_TRY(h->d_stats.reserve(2 * nq));
+    HIP_TRY(hipMemcpyAsync(h->d_q.p, queries, nq * h->dim * 4, hipMemcpyHostToDevice, h->stream));
+    if (ef) HIP_TRY(hipMemcpyAsync(h->d_ef.p, ef, nq * 4, hipMemcpyHostToDevice, h->stream));
+    if (k) HIP_TRY(hipMemcpyAsync(h->d_k.p, k, nq * 4, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(hipEventRecord(h->ev0, h->stream));
+    st = run_batch(h, h->d_q.p, nq, ef ? h->d_ef.p : nullptr, k ? h->d_k.p : nullptr, ef_max, kmax,
+                   order, h->d_ids.p, dists ? h->d_dist.p : nullptr,
+                   stats ? h->d_stats.p : nullptr, h->stream);
+    if (st != LQG_OK) return st;
+    HIP_TRY(hipEventRecord(h->ev1, h->stream));
+    HIP_TRY(hipMemcpyAsync(results, h->d_ids.p, nq * kmax * 4, hipMemcpyDeviceToHost, h->stream));
+    if (dists)
+        HIP_TRY(hipMemcpyAsync(dists, h->d_dist.p, nq * kmax * 4, hipMemcpyDeviceToHost, h->stream));
+    std::vector<uint32_t> st_host;
+    if (stats) {
+        st_host.resize(2 * nq);
+        HIP_TRY(hipMemcpyAsync(st_host.data(), h->d_stats.p, 2 * nq * 4, hipMemcpyDeviceToHost,
+                               h->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (stats) {
+        stats->queries = nq;
+        stats->expansions = 0;
+        stats->evaluations = 0;
+        for (uint64_t i = 0; i < nq; ++i) {
+            stats->expansions += st_host[2 * i];
+            stats->evaluations += st_host[2 * i + 1];
+        }
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        stats->device_ms = ms;
+    }
+    return LQG_OK;
+}
+
+lqg_status lqg_search_batch_device(lqg_index* h, const float* d_queries, uint64_t nq,
+                                   const uint32_t* d_ef, const uint32_t* d_k, uint32_t ef_max,
+                                   uint32_t kmax, lqg_order order, uint32_t* d_results,
+                                   float* d_dists, uint32_t* d_stats, void* stream) {
+    if (!h || !d_queries || !d_results) return fail(LQG_ERR_ARG, "null argument");
+    if (nq == 0 || nq > 0xFFFFFFFFull) return fail(LQG_ERR_ARG, "nq must be in [1, 2^32)");
+    if (kmax == 0) return fail(LQG_ERR_ARG, "kmax must be positive");
+    if (!valid_order(order)) return fail(LQG_ERR_ARG, "unknown order");
+    if (ef_max == 0) {
+        if (d_ef) return fail(LQG_ERR_ARG, "ef_max must bound the per-query ef array");
+        if (!h->ef_set) return fail(LQG_ERR_STATE, "set_ef() or ef_max required");
+        ef_max = h->ef;
+    }
+    if (ef_class(ef_max) == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported");
+    lqg_status st = ensure_device(h);
+    if (st != LQG_OK) return st;
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->stream;
+    return run_batch(h, d_queries, nq, d_ef, d_k, ef_max, kmax, order, d_results, d_dists, d_stats,
+                     s);
+}
+
+lqg_status lqg_info(const lqg_index* h, uint64_t* num_elements, uint64_t* dimension,
+                    uint64_t* degree_bound, uint32_t* entry_point, uint32_t* ef) {
+    if (!h) return fail(LQG_ERR_ARG, "null index");
+    if (num_elements) *num_elements = h->n;
+    if (dimension) *dimension = h->dim;
+    if (degree_bound) *degree_bound = h->deg;
+    if (entry_point) *entry_point = h->ep;
+    if (ef) *ef = h->ef_set ? h->ef : 0;
+    return LQG_OK;
+}
+
+lqg_status lqg_get_graph(const lqg_index* h, uint32_t* neighbors) {
+    if (!h || !neighbors) return fail(LQG_ERR_ARG, "null argument");
+    if (!h->have_data) return fail(LQG_ERR_STATE, "index has no data");
+    memcpy(neighbors, h->nbr.data(), h->nbr.size() * 4);
+    return LQG_OK;
+}
+
+lqg_status lqg_build_index(lqg_index* h, const float* data, uint64_t num_points,
+                           uint64_t dimension, uint32_t ef_build, uint32_t num_iter,
+                           uint32_t num_thread) {
+    (void)data;
+    (void)ef_build;
+    (void)num_iter;
+    (void)num_thread;
+    if (!h) return fail(LQG_ERR_ARG, "null index");
+    if (num_points != h->n || dimension != h->dim)
+        return fail(LQG_ERR_ARG, "The shape of data is different with initialization!");
+    return fail(LQG_ERR_UNSUPPORTED, "build_index is not available yet");
+}
+
+}  // extern "C"

@@ -1,0 +1,56 @@
+// lqg_internal.h -- declarations shared by the host library and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lqg {
+
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr int kWave = 64;
+
+// Device view of an index for the SQ8 search (the reference's live path,
+// qg.hpp:298-344).  qdata rows are `dim` bytes, neighbour rows `deg` ids.
+struct DevIndex {
+    const uint8_t* qdata;   // N x dim SQ8 codes (file section 3)
+    const uint32_t* nbr;    // N x deg neighbour ids (file section 2, float offset D)
+    const float* sq8_min;   // dim
+    const float* sq8_scale; // dim
+    uint32_t n, dim, deg, ep;
+};
+
+// One batch of queries for the search kernel.
+struct SearchJob {
+    const float* queries;   // nq x dim
+    const uint32_t* ef;     // per query, or null -> ef_all
+    const uint32_t* k;      // per query, or null -> k_all
+    uint32_t nq, ef_all, k_all, kstride;
+    int order;              // 0 reference, 1 nearest
+    uint32_t* out_ids;      // nq x kstride
+    float* out_d;           // nq x kstride or null
+    uint32_t* out_stats;    // nq x 2 (E, V) or null
+    uint32_t* counter;      // persistent work queue head (device), zeroed before launch
+};
+
+// Launch geometry chosen by the host (autotuner or defaults).
+struct LaunchCfg {
+    int ef_cap;          // pool capacity class: 64, 128, 256, 512 or 1024
+    int waves_per_group; // waves per workgroup
+    int cache_bits;      // log2 of the per-wave visited cache
+    int groups;          // workgroups in the persistent grid (0 = fill the device)
+};
+
+// Returns LDS bytes one workgroup needs for this configuration.
+size_t search_lds_bytes(const DevIndex& ix, const LaunchCfg& cfg);
+
+// Enqueues the search kernel (queries pulled from job.counter).
+hipError_t launch_search(const DevIndex& ix, const SearchJob& job, const LaunchCfg& cfg,
+                         hipStream_t stream);
+
+// Zeroes the work-queue counter with a kernel (capturable in a HIP graph).
+hipError_t launch_reset_counter(uint32_t* counter, hipStream_t stream);
+
+// Number of workgroups that fill the device for this configuration.
+int search_full_grid(const DevIndex& ix, const LaunchCfg& cfg);
+
+}  // namespace lqg

@@ -1,0 +1,236 @@
+"""ctypes front end of the CPU oracle (oracle/lqg_oracle.c) and of the index
+file layout.  TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check
+and by the CPU-baseline leg of bench.py, never by the library.
+
+Index file body (qg.hpp:206-224; third/svs/array.hpp:109-116; shapes from
+qg.hpp:183-193,420-436), no header:
+
+    u32 entry point
+    N x (D + deg) float32 rows: D raw floats, then deg u32 neighbour ids
+    N x D uint8 SQ8 codes
+    D x {float32 min, float32 scale}
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblqg_oracle.so")
+REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
+FILL = 0xFFFFFFFF
+
+
+def build(force: bool = False) -> None:
+    """Compile the C restatement (and oracle/_ref where /root/reference exists)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        L.orc_sq8_dequantize.argtypes = [P, C.c_uint32, P, P, C.c_int]
+        L.orc_l2_sqr.argtypes = [P, P, C.c_uint32]
+        L.orc_l2_sqr.restype = C.c_float
+        L.orc_search_heap.argtypes = [P, P, C.c_uint32, C.c_uint32, P, P, P, P, P]
+        L.orc_search_canon.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_int, P, P, P, P, P, P]
+        L.orc_search_canon_batch.argtypes = [P, P, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int,
+                                             P, P, P, C.c_int]
+        L.orc_search_heap_batch.argtypes = [P, P, C.c_uint64, C.c_uint32, C.c_uint32, P, C.c_int]
+        L.orc_bruteforce_fp64.argtypes = [P, C.c_uint64, C.c_uint32, P, C.c_uint64, C.c_uint32,
+                                          P, P, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class _OrcIndex(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("dim", C.c_uint32), ("deg", C.c_uint32), ("ep", C.c_uint32),
+                ("nbr", C.c_void_p), ("qdata", C.c_void_p), ("params", C.c_void_p),
+                ("fused", C.c_int)]
+
+
+class IndexArrays:
+    """The three arrays the search reads, plus the raw vectors and entry point."""
+
+    def __init__(self, ep, raw, nbr, qdata, params, fused: bool = True):
+        self.ep = int(ep)
+        self.raw = np.ascontiguousarray(raw, np.float32)
+        self.nbr = np.ascontiguousarray(nbr, np.uint32)
+        self.qdata = np.ascontiguousarray(qdata, np.uint8)
+        self.params = np.ascontiguousarray(params, np.float32).reshape(-1, 2)
+        self.n, self.dim = self.qdata.shape
+        self.deg = self.nbr.shape[1]
+        self.fused = bool(fused)
+        self._s = _OrcIndex(self.n, self.dim, self.deg, self.ep, self.nbr.ctypes.data,
+                            self.qdata.ctypes.data, self.params.ctypes.data, int(self.fused))
+
+    @property
+    def ref(self):
+        return C.byref(self._s)
+
+    def with_rounding(self, fused: bool) -> "IndexArrays":
+        return IndexArrays(self.ep, self.raw, self.nbr, self.qdata, self.params, fused)
+
+
+def file_size(n: int, d: int, deg: int) -> int:
+    return 4 + n * (d + deg) * 4 + n * d + 8 * d
+
+
+def read_index(path: str, n: int, d: int, deg: int, fused: bool = True) -> IndexArrays:
+    expect = file_size(n, d, deg)
+    got = os.path.getsize(path)
+    if got != expect:
+        raise ValueError(f"{path}: {got} bytes, expected {expect} for N={n} D={d} deg={deg}")
+    buf = np.fromfile(path, dtype=np.uint8)
+    ep = int(buf[:4].view(np.uint32)[0])
+    off = 4
+    rows = buf[off:off + n * (d + deg) * 4].view(np.float32).reshape(n, d + deg)
+    off += n * (d + deg) * 4
+    qdata = buf[off:off + n * d].reshape(n, d)
+    off += n * d
+    params = buf[off:off + 8 * d].view(np.float32).reshape(d, 2)
+    raw = rows[:, :d]
+    nbr = rows[:, d:].view(np.uint32)
+    return IndexArrays(ep, raw, nbr, qdata, params, fused)
+
+
+def write_index(path: str, ix: IndexArrays) -> None:
+    n, d, deg = ix.n, ix.dim, ix.deg
+    rows = np.empty((n, d + deg), np.float32)
+    rows[:, :d] = ix.raw
+    rows[:, d:] = ix.nbr.view(np.float32)
+    with open(path, "wb") as f:
+        f.write(np.uint32(ix.ep).tobytes())
+        f.write(rows.tobytes())
+        f.write(ix.qdata.tobytes())
+        f.write(ix.params.astype(np.float32).tobytes())
+
+
+def sq8_quantize(data: np.ndarray):
+    """Per-dimension min/scale codes as quantization/sq8.hpp:11-26 computes them
+    in the reference's -Ofast build.
+
+    ``scale = (max - min) * fl(1/255)`` (the build turns the division by the
+    constant 255.0f into a multiplication by its float reciprocal), then codes
+    ``round((v - min) / scale)`` with a true float32 division and
+    round-half-away-from-zero (std::round).  Both facts are fixed by the
+    reference-built index files in tests/golden.  A dimension with max == min
+    gets scale 0 and the reference's 0/0 -> NaN -> cast; here such codes are 0.
+    """
+    data = np.ascontiguousarray(data, np.float32)
+    mn = data.min(axis=0)
+    mx = data.max(axis=0)
+    scale = ((mx - mn) * (np.float32(1.0) / np.float32(255.0))).astype(np.float32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = (data - mn) / scale
+    r = np.where(np.isfinite(r), r, 0.0).astype(np.float32)
+    t = np.trunc(r)
+    q = t + np.sign(r) * (np.abs(r - t) >= np.float32(0.5))
+    params = np.stack([mn, scale], axis=1).astype(np.float32)
+    return np.clip(q, 0, 255).astype(np.uint8), params
+
+
+def sq8_dequantize(codes, params, fused: bool = True) -> np.ndarray:
+    codes = np.ascontiguousarray(codes, np.uint8)
+    params = np.ascontiguousarray(params, np.float32)
+    out = np.empty(codes.shape, np.float32)
+    L = lib()
+    rows = codes.reshape(-1, codes.shape[-1])
+    o = out.reshape(rows.shape)
+    for i in range(rows.shape[0]):
+        L.orc_sq8_dequantize(_p(rows[i]), rows.shape[1], _p(params), _p(o[i]), int(fused))
+    return out
+
+
+def l2_sqr(a, b) -> float:
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return float(lib().orc_l2_sqr(_p(a), _p(b), a.shape[0]))
+
+
+def search_heap(ix: IndexArrays, query, ef: int, k: int):
+    """Reference H1 replica: (results[k], pool_ids, pool_d, (E, V))."""
+    q = np.ascontiguousarray(query, np.float32)
+    res = np.empty(k, np.uint32)
+    pid = np.empty(ef, np.uint32)
+    pd = np.empty(ef, np.float32)
+    pn = np.zeros(1, np.uint32)
+    st = np.zeros(2, np.uint64)
+    rc = lib().orc_search_heap(ix.ref, _p(q), ef, k, _p(res), _p(pid), _p(pd), _p(pn), _p(st))
+    if rc != 0:
+        raise RuntimeError(f"orc_search_heap failed ({rc})")
+    n = int(pn[0])
+    return res, pid[:n].copy(), pd[:n].copy(), (int(st[0]), int(st[1]))
+
+
+def search_canon(ix: IndexArrays, query, ef: int, k: int, order: str = "nearest"):
+    """Total-order H1: (results[k], dists[k], pool_ids, pool_d, (E, V))."""
+    q = np.ascontiguousarray(query, np.float32)
+    res = np.empty(k, np.uint32)
+    dis = np.empty(k, np.float32)
+    pid = np.empty(ef, np.uint32)
+    pd = np.empty(ef, np.float32)
+    pn = np.zeros(1, np.uint32)
+    st = np.zeros(2, np.uint64)
+    rc = lib().orc_search_canon(ix.ref, _p(q), ef, k, 0 if order == "reference" else 1,
+                                _p(res), _p(dis), _p(pid), _p(pd), _p(pn), _p(st))
+    if rc != 0:
+        raise RuntimeError(f"orc_search_canon failed ({rc})")
+    n = int(pn[0])
+    return res, dis, pid[:n].copy(), pd[:n].copy(), (int(st[0]), int(st[1]))
+
+
+def search_canon_batch(ix: IndexArrays, queries, ef: int, k: int, order: str = "nearest",
+                       threads: int = 0):
+    Q = np.ascontiguousarray(queries, np.float32)
+    nq = Q.shape[0]
+    res = np.empty((nq, k), np.uint32)
+    dis = np.empty((nq, k), np.float32)
+    st = np.zeros((nq, 2), np.uint64)
+    rc = lib().orc_search_canon_batch(ix.ref, _p(Q), nq, ef, k, 0 if order == "reference" else 1,
+                                      _p(res), _p(dis), _p(st), threads)
+    if rc != 0:
+        raise RuntimeError("orc_search_canon_batch failed")
+    return res, dis, st
+
+
+def search_heap_batch(ix: IndexArrays, queries, ef: int, k: int, threads: int = 0):
+    Q = np.ascontiguousarray(queries, np.float32)
+    res = np.empty((Q.shape[0], k), np.uint32)
+    if lib().orc_search_heap_batch(ix.ref, _p(Q), Q.shape[0], ef, k, _p(res), threads) != 0:
+        raise RuntimeError("orc_search_heap_batch failed")
+    return res
+
+
+def bruteforce_fp64(base, queries, k: int, threads: int = 0):
+    B = np.ascontiguousarray(base, np.float32)
+    Q = np.ascontiguousarray(queries, np.float32)
+    ids = np.empty((Q.shape[0], k), np.uint32)
+    dis = np.empty((Q.shape[0], k), np.float64)
+    if lib().orc_bruteforce_fp64(_p(B), B.shape[0], B.shape[1], _p(Q), Q.shape[0], k, _p(ids),
+                                 _p(dis), threads) != 0:
+        raise RuntimeError("orc_bruteforce_fp64 failed")
+    return ids, dis
+
+
+def recall_at_k(found: np.ndarray, gt: np.ndarray, k: int) -> float:
+    """|found[:k] & gt[:k]| / k averaged over queries (reproduce/run.py:31-40)."""
+    tot = 0
+    for f, g in zip(found, gt):
+        tot += len(set(int(x) for x in f[:k]) & set(int(x) for x in g[:k]))
+    return tot / (k * len(found))

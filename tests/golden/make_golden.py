@@ -1,0 +1,109 @@
+"""Writes the golden fixtures in tests/golden/ from the reference's own code.
+
+Runs only where the reference tree is present: it drives oracle/_ref/ref_driver
+(built by ``make -C oracle ref`` from the reference's headers) and stores
+inputs and outputs as data files.  Reference builds are not reproducible
+(the rotator and the random initial graph draw from std::random_device,
+utils/rotator.hpp:37-43, utils/tools.hpp:8-15), so the index FILES are the
+fixtures, not a recipe.
+
+Fixtures written (all small):
+  <name>.index           reference-built index file (QGBuilder, save_index)
+  <name>.npz             base/queries shapes, queries, and the reference's
+                         search() ids for each ef in EFS at k=10
+  funcs_d<D>.npz         sq8_dequantize + l2_sqr vectors for several D
+
+Usage:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from liteqg_amd import datasets  # noqa: E402
+
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+EFS = (10, 20, 50, 100)
+K = 10
+
+# name: (process kind, N, D, deg, build EF, queries, generator kwargs, seed)
+INDEXES = {
+    "g_uniform_n2000_d64": ("uniform", 2000, 64, 32, 200, 200, {}, 11),
+    "g_intmix_n1200_d128": ("int_mix", 1200, 128, 32, 200, 200, dict(centres=8, sigma=20.0), 12),
+    "g_unitmix_n1000_d96_deg64": ("unit_mix", 1000, 96, 64, 200, 150,
+                                  dict(centres=8, sigma=0.3), 13),
+    # values in {0,1,2,3}: squared distances are small integers, so exact
+    # ties are everywhere and the heap's tie order decides the output
+    "g_ties_n1500_d64": ("ties", 1500, 64, 32, 100, 200, {}, 14),
+}
+FUNC_DIMS = (64, 96, 128, 768, 960, 80)
+
+
+def run(*args):
+    subprocess.run([DRIVER, *map(str, args)], check=True, stdout=subprocess.DEVNULL)
+
+
+def make_index(name, kind, n, d, deg, ef_build, nq, kw, seed):
+    if kind == "ties":
+        rng = np.random.default_rng(seed)
+        base = rng.integers(0, 4, size=(n, d)).astype(np.float32)
+        queries = rng.integers(0, 4, size=(nq, d)).astype(np.float32)
+    else:
+        base, queries = datasets.make(kind, n, d, nq, seed=seed, **kw)
+    with tempfile.TemporaryDirectory() as tmp:
+        bpath = os.path.join(tmp, "base.f32")
+        qpath = os.path.join(tmp, "q.f32")
+        base.tofile(bpath)
+        queries.tofile(qpath)
+        ipath = os.path.join(HERE, name + ".index")
+        run("build", bpath, n, d, deg, ef_build, 3, 8, ipath)
+        out = {}
+        for ef in EFS:
+            opath = os.path.join(tmp, f"o{ef}.u32")
+            run("search", ipath, n, d, deg, qpath, nq, ef, K, opath)
+            out[f"ids_ef{ef}"] = np.fromfile(opath, np.uint32).reshape(nq, K)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), queries=queries,
+                        shape=np.array([n, d, deg], np.int64), k=np.int64(K),
+                        efs=np.array(EFS, np.int64), **out)
+
+
+def make_funcs(d, m=None, seed=5):
+    m = m or (64 if d <= 128 else 12)
+    rng = np.random.default_rng(seed + d)
+    codes = rng.integers(0, 256, size=(m, d), dtype=np.uint8)
+    mn = rng.uniform(-2.0, 2.0, size=d).astype(np.float32)
+    sc = rng.uniform(1e-4, 0.05, size=d).astype(np.float32)
+    params = np.stack([mn, sc], axis=1).astype(np.float32)
+    q = (mn + codes.astype(np.float32) * sc + rng.normal(0, 0.3, size=(m, d))).astype(np.float32)
+    with tempfile.TemporaryDirectory() as tmp:
+        p = [os.path.join(tmp, x) for x in ("c.u8", "p.f32", "q.f32", "dec.f32", "l2.f32")]
+        codes.tofile(p[0])
+        params.tofile(p[1])
+        q.tofile(p[2])
+        run("funcs", p[0], p[1], p[2], d, m, p[3], p[4])
+        dec = np.fromfile(p[3], np.float32).reshape(m, d)
+        l2 = np.fromfile(p[4], np.float32)
+    np.savez_compressed(os.path.join(HERE, f"funcs_d{d}.npz"), codes=codes, params=params,
+                        queries=q, decoded=dec, l2=l2)
+
+
+def main():
+    if not os.path.exists(DRIVER):
+        sys.exit("oracle/_ref/ref_driver missing: run `make -C oracle ref` first")
+    for d in FUNC_DIMS:
+        make_funcs(d)
+    for name, spec in INDEXES.items():
+        make_index(name, *spec)
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,156 @@
+"""The C ABI: symbols, argument validation, file handling -- no device calls."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+HEADER = os.path.join(ROOT, "include", "liteqg.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:lqg_status|const char\*)\s+(lqg_\w+)\s*\(", text, re.M)))
+
+
+@pytest.fixture(scope="module")
+def L():
+    from liteqg_amd import _lib
+
+    return _lib
+
+
+def test_library_exports_every_declared_symbol(L):
+    lib = L.lib()
+    syms = declared_symbols()
+    assert "lqg_search" in syms and "lqg_search_batch_device" in syms
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(L.EXPORTED)
+
+
+def test_library_is_built_for_gfx950_only():
+    data = open(os.path.join(ROOT, "liteqg_amd", "liblqg.so"), "rb").read()
+    assert b"gfx950" in data
+    for other in (b"gfx942", b"gfx90a", b"gfx1100"):
+        assert b"amdgcn-amd-amdhsa--" + other not in data
+
+
+def test_constructor_validation():
+    from liteqg_amd.symphonyqg import Index
+
+    Index("QG", "L2", 100, 16, 32)
+    Index("QG", "L2", 100, 16, degree_bound=64)
+    with pytest.raises(ValueError, match="L2"):
+        Index("QG", "IP", 100, 16)
+    with pytest.raises(ValueError, match="multiple of 32"):
+        Index("QG", "L2", 100, 16, 48)
+    with pytest.raises(ValueError, match="multiple of 32"):
+        Index("QG", "L2", 100, 16, 0)
+    with pytest.raises(ValueError, match="not supported"):
+        Index("HNSW", "L2", 100, 16)
+    with pytest.raises(ValueError):
+        Index("QG", "L2", 0, 16)
+
+
+def test_keyword_names_match_reference():
+    """bindings.cpp:104-124 keyword names."""
+    import inspect
+
+    from liteqg_amd.symphonyqg import Index
+
+    assert list(inspect.signature(Index.__init__).parameters)[1:] == [
+        "index_type", "metric", "num_elements", "dimension", "degree_bound"]
+    assert list(inspect.signature(Index.load).parameters)[1:] == ["filename"]
+    assert list(inspect.signature(Index.save).parameters)[1:] == ["filename"]
+    assert list(inspect.signature(Index.set_ef).parameters)[1:] == ["EF"]
+    assert list(inspect.signature(Index.build_index).parameters)[1:] == [
+        "data", "EF", "num_iter", "num_thread"]
+    assert list(inspect.signature(Index.search).parameters)[1:] == ["query", "k"]
+    import symphonyqg
+
+    assert symphonyqg.Index is Index
+
+
+def test_load_save_roundtrip_is_byte_identical(tmp_path):
+    from liteqg_amd.symphonyqg import Index
+
+    src = f"{GOLDEN}/g_intmix_n1200_d128.index"
+    ix = Index("QG", "L2", 1200, 128, 32)
+    ix.load(src)
+    info = ix.info()
+    assert info["entry_point"] == int(np.fromfile(src, np.uint32, 1)[0])
+    out = tmp_path / "copy.index"
+    ix.save(str(out))
+    assert out.read_bytes() == open(src, "rb").read()
+
+
+def test_load_rejects_wrong_shapes(tmp_path):
+    from liteqg_amd.symphonyqg import Index, LqgError
+
+    src = f"{GOLDEN}/g_intmix_n1200_d128.index"
+    with pytest.raises(LqgError, match="expected"):
+        Index("QG", "L2", 1201, 128, 32).load(src)
+    with pytest.raises(LqgError, match="expected"):
+        Index("QG", "L2", 1200, 128, 64).load(src)
+    with pytest.raises(LqgError, match="cannot open"):
+        Index("QG", "L2", 1200, 128, 32).load(str(tmp_path / "missing.index"))
+    trunc = tmp_path / "trunc.index"
+    trunc.write_bytes(open(src, "rb").read()[:-1])
+    with pytest.raises(LqgError, match="expected"):
+        Index("QG", "L2", 1200, 128, 32).load(str(trunc))
+
+
+def test_load_rejects_out_of_range_ids(tmp_path):
+    from liteqg_amd.symphonyqg import Index, LqgError
+
+    src = open(f"{GOLDEN}/g_uniform_n2000_d64.index", "rb").read()
+    bad = bytearray(src)
+    off = 4 + 64 * 4  # first neighbour id of row 0
+    bad[off:off + 4] = np.uint32(2000).tobytes()
+    p = tmp_path / "bad.index"
+    p.write_bytes(bytes(bad))
+    with pytest.raises(LqgError, match="out of range"):
+        Index("QG", "L2", 2000, 64, 32).load(str(p))
+
+
+def test_call_order_errors():
+    from liteqg_amd.symphonyqg import Index, LqgError
+
+    ix = Index("QG", "L2", 1200, 128, 32)
+    with pytest.raises(LqgError, match="set_ef"):
+        ix.search(np.zeros(128, np.float32), 10)
+    with pytest.raises(LqgError):
+        ix.set_ef(0)
+    with pytest.raises(LqgError, match="1024"):
+        ix.set_ef(5000)
+    ix.set_ef(10)
+    with pytest.raises(LqgError, match="no data"):
+        ix.search(np.zeros(128, np.float32), 10)
+    with pytest.raises(LqgError, match="no data"):
+        ix.save("/tmp/never_written.index")
+    with pytest.raises(ValueError):
+        ix.search(np.zeros(5, np.float32), 10)
+    with pytest.raises(ValueError):
+        ix.set_order("farthest")
+    ix.load(f"{GOLDEN}/g_intmix_n1200_d128.index")
+    with pytest.raises(LqgError, match="positive"):
+        ix.search(np.zeros(128, np.float32), 0)
+    q = np.zeros(128, np.float32)
+    q[3] = np.nan
+    with pytest.raises(LqgError, match="non-finite"):
+        ix.search(q, 10)
+    with pytest.raises(LqgError, match="kmax"):
+        ix.search_batch(np.zeros((3, 128), np.float32), np.array([1, 2, 0]))
+
+
+def test_build_index_shape_check():
+    from liteqg_amd.symphonyqg import Index, LqgError
+
+    ix = Index("QG", "L2", 100, 16, 32)
+    with pytest.raises(LqgError, match="shape"):
+        ix.build_index(np.zeros((99, 16), np.float32), 50)
+    with pytest.raises(ValueError, match="shape"):
+        ix.build_index(np.zeros((2, 3, 4), np.float32), 50)
