@@ -12,12 +12,13 @@
 #include <string>
 #include <vector>
 
-#include "../../include/liteqg.h"
-#include "lqg_internal.h"
+#include "lqg_index.h"
+
+namespace lqg {
 
 namespace {
-
 thread_local std::string g_err;
+}
 
 lqg_status fail(lqg_status s, const std::string& msg) {
     g_err = msg;
@@ -28,31 +29,7 @@ lqg_status hip_fail(hipError_t e, const char* what) {
     return fail(LQG_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define HIP_TRY(expr)                                   \
-    do {                                                \
-        hipError_t e_ = (expr);                         \
-        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
-    } while (0)
-
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t cap = 0;  // elements
-    hipError_t reserve(size_t n) {
-        if (n <= cap) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(n, 1) * sizeof(T));
-        if (e == hipSuccess) cap = n;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
+const char* last_error() { return g_err.c_str(); }
 
 int ef_class(uint32_t ef) {
     const int classes[] = {64, 128, 256, 512, 1024};
@@ -61,87 +38,46 @@ int ef_class(uint32_t ef) {
     return 0;
 }
 
-}  // namespace
-
-struct lqg_index {
-    uint64_t n = 0, dim = 0, deg = 0;
-    uint32_t ep = 0;
-    uint32_t ef = 0;
-    bool ef_set = false;
-    bool have_data = false;  // loaded or built
-    lqg_order order = LQG_ORDER_REFERENCE;
-
-    // host copy: exactly the file's content, split by section
-    std::vector<float> raw;       // n x dim
-    std::vector<uint32_t> nbr;    // n x deg
-    std::vector<uint8_t> qdata;   // n x dim
-    std::vector<float> sq8;       // dim x {min, scale}
-
-    // device copy
-    bool on_device = false;
-    hipStream_t stream = nullptr;
-    DevBuf<uint8_t> d_qdata;
-    DevBuf<uint32_t> d_nbr;
-    DevBuf<float> d_min, d_scale;
-    // per-call scratch (grow only)
-    DevBuf<float> d_q;
-    DevBuf<uint32_t> d_ids, d_ef, d_k, d_stats, d_counter;
-    DevBuf<float> d_dist;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-
-    lqg::DevIndex dev() const {
-        lqg::DevIndex ix;
-        ix.qdata = d_qdata.p;
-        ix.nbr = d_nbr.p;
-        ix.sq8_min = d_min.p;
-        ix.sq8_scale = d_scale.p;
-        ix.n = static_cast<uint32_t>(n);
-        ix.dim = static_cast<uint32_t>(dim);
-        ix.deg = static_cast<uint32_t>(deg);
-        ix.ep = ep;
-        return ix;
-    }
-};
-
-namespace {
-
 lqg_status ensure_device(lqg_index* h) {
     if (h->on_device) return LQG_OK;
-    if (!h->have_data) return fail(LQG_ERR_STATE, "index has no data: call load() or build_index() first");
+    if (!h->have_data)
+        return fail(LQG_ERR_STATE, "index has no data: call load() or build_index() first");
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev == 0) return fail(LQG_ERR_DEVICE, "no HIP device available");
     int dev = 0;
-    HIP_TRY(hipGetDevice(&dev));
+    LQG_HIP_TRY(hipGetDevice(&dev));
     hipDeviceProp_t prop;
-    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    LQG_HIP_TRY(hipGetDeviceProperties(&prop, dev));
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(LQG_ERR_DEVICE, std::string("device is ") + prop.gcnArchName +
                                         ", this library is built for gfx950 only");
-    if (!h->stream) HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    if (!h->ev0) HIP_TRY(hipEventCreate(&h->ev0));
-    if (!h->ev1) HIP_TRY(hipEventCreate(&h->ev1));
+    if (!h->stream) LQG_HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    if (!h->ev0) LQG_HIP_TRY(hipEventCreate(&h->ev0));
+    if (!h->ev1) LQG_HIP_TRY(hipEventCreate(&h->ev1));
     const size_t nd = h->n * h->dim, ng = h->n * h->deg;
-    HIP_TRY(h->d_qdata.reserve(nd));
-    HIP_TRY(h->d_nbr.reserve(ng));
-    HIP_TRY(h->d_min.reserve(h->dim));
-    HIP_TRY(h->d_scale.reserve(h->dim));
+    LQG_HIP_TRY(h->d_qdata.reserve(nd));
+    LQG_HIP_TRY(h->d_nbr.reserve(ng));
+    LQG_HIP_TRY(h->d_min.reserve(h->dim));
+    LQG_HIP_TRY(h->d_scale.reserve(h->dim));
     std::vector<float> mn(h->dim), sc(h->dim);
     for (uint64_t i = 0; i < h->dim; ++i) {
         mn[i] = h->sq8[2 * i];
         sc[i] = h->sq8[2 * i + 1];
     }
-    HIP_TRY(hipMemcpy(h->d_qdata.p, h->qdata.data(), nd, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(h->d_nbr.p, h->nbr.data(), ng * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(h->d_min.p, mn.data(), h->dim * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(h->d_scale.p, sc.data(), h->dim * 4, hipMemcpyHostToDevice));
-    HIP_TRY(h->d_counter.reserve(1));
+    LQG_HIP_TRY(hipMemcpy(h->d_qdata.p, h->qdata.data(), nd, hipMemcpyHostToDevice));
+    LQG_HIP_TRY(hipMemcpy(h->d_nbr.p, h->nbr.data(), ng * 4, hipMemcpyHostToDevice));
+    LQG_HIP_TRY(hipMemcpy(h->d_min.p, mn.data(), h->dim * 4, hipMemcpyHostToDevice));
+    LQG_HIP_TRY(hipMemcpy(h->d_scale.p, sc.data(), h->dim * 4, hipMemcpyHostToDevice));
+    LQG_HIP_TRY(h->d_counter.reserve(1));
     h->on_device = true;
     return LQG_OK;
 }
 
-lqg::LaunchCfg default_cfg(uint32_t ef_max, uint64_t nq) {
-    lqg::LaunchCfg c;
+namespace {
+
+LaunchCfg default_cfg(uint32_t ef_max, uint64_t nq) {
+    LaunchCfg c;
     c.ef_cap = ef_class(ef_max);
     c.waves_per_group = nq >= 4 ? 4 : 1;
     c.cache_bits = 10;
@@ -149,18 +85,14 @@ lqg::LaunchCfg default_cfg(uint32_t ef_max, uint64_t nq) {
     return c;
 }
 
-lqg_status check_queries(const float* q, uint64_t count) {
-    for (uint64_t i = 0; i < count; ++i)
-        if (!std::isfinite(q[i])) return fail(LQG_ERR_ARG, "query contains a non-finite value");
-    return LQG_OK;
-}
+}  // namespace
 
 lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
                      const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, lqg_order order,
                      uint32_t* d_results, float* d_dists, uint32_t* d_stats, hipStream_t s) {
-    const int cls = ef_class(ef_max);
-    if (cls == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported on the device");
-    lqg::SearchJob job;
+    if (ef_class(ef_max) == 0)
+        return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported on the device");
+    SearchJob job;
     job.queries = d_queries;
     job.ef = d_ef;
     job.k = d_k;
@@ -173,9 +105,22 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
     job.out_d = d_dists;
     job.out_stats = d_stats;
     job.counter = h->d_counter.p;
-    const lqg::LaunchCfg cfg = default_cfg(ef_max, nq);
-    HIP_TRY(lqg::launch_reset_counter(h->d_counter.p, s));
-    HIP_TRY(lqg::launch_search(h->dev(), job, cfg, s));
+    const LaunchCfg cfg = default_cfg(ef_max, nq);
+    LQG_HIP_TRY(launch_reset_counter(h->d_counter.p, s));
+    LQG_HIP_TRY(launch_search(h->dev(), job, cfg, s));
+    return LQG_OK;
+}
+
+}  // namespace lqg
+
+namespace {
+
+using lqg::ef_class;
+using lqg::fail;
+
+lqg_status check_queries(const float* q, uint64_t count) {
+    for (uint64_t i = 0; i < count; ++i)
+        if (!std::isfinite(q[i])) return fail(LQG_ERR_ARG, "query contains a non-finite value");
     return LQG_OK;
 }
 
@@ -183,9 +128,11 @@ bool valid_order(lqg_order o) { return o == LQG_ORDER_REFERENCE || o == LQG_ORDE
 
 }  // namespace
 
+#define HIP_TRY LQG_HIP_TRY
+
 extern "C" {
 
-const char* lqg_last_error(void) { return g_err.c_str(); }
+const char* lqg_last_error(void) { return lqg::last_error(); }
 
 lqg_status lqg_create(const char* index_type, const char* metric, uint64_t num_elements,
                       uint64_t dimension, uint64_t degree_bound, lqg_index** out) {
@@ -339,12 +286,12 @@ lqg_status lqg_search(lqg_index* h, const float* query, uint32_t k, uint32_t* re
     if (k == 0) return fail(LQG_ERR_ARG, "k must be positive");
     lqg_status st = check_queries(query, h->dim);
     if (st != LQG_OK) return st;
-    st = ensure_device(h);
+    st = lqg::ensure_device(h);
     if (st != LQG_OK) return st;
     HIP_TRY(h->d_q.reserve(h->dim));
     HIP_TRY(h->d_ids.reserve(k));
     HIP_TRY(hipMemcpyAsync(h->d_q.p, query, h->dim * 4, hipMemcpyHostToDevice, h->stream));
-    st = run_batch(h, h->d_q.p, 1, nullptr, nullptr, h->ef, k, h->order, h->d_ids.p, nullptr,
+    st = lqg::run_batch(h, h->d_q.p, 1, nullptr, nullptr, h->ef, k, h->order, h->d_ids.p, nullptr,
                    nullptr, h->stream);
     if (st != LQG_OK) return st;
     HIP_TRY(hipMemcpyAsync(results, h->d_ids.p, k * 4ull, hipMemcpyDeviceToHost, h->stream));
@@ -374,7 +321,7 @@ lqg_status lqg_search_batch(lqg_index* h, const float* queries, uint64_t nq, con
     if (ef_class(ef_max) == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported");
     lqg_status st = check_queries(queries, nq * h->dim);
     if (st != LQG_OK) return st;
-    st = ensure_device(h);
+    st = lqg::ensure_device(h);
     if (st != LQG_OK) return st;
     HIP_TRY(h->d_q.reserve(nq * h->dim));
     HIP_TRY(h->d_ids.reserve(nq * kmax));
@@ -386,7 +333,7 @@ lqg_status lqg_search_batch(lqg_index* h, const float* queries, uint64_t nq, con
     if (ef) HIP_TRY(hipMemcpyAsync(h->d_ef.p, ef, nq * 4, hipMemcpyHostToDevice, h->stream));
     if (k) HIP_TRY(hipMemcpyAsync(h->d_k.p, k, nq * 4, hipMemcpyHostToDevice, h->stream));
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    st = run_batch(h, h->d_q.p, nq, ef ? h->d_ef.p : nullptr, k ? h->d_k.p : nullptr, ef_max, kmax,
+    st = lqg::run_batch(h, h->d_q.p, nq, ef ? h->d_ef.p : nullptr, k ? h->d_k.p : nullptr, ef_max, kmax,
                    order, h->d_ids.p, dists ? h->d_dist.p : nullptr,
                    stats ? h->d_stats.p : nullptr, h->stream);
     if (st != LQG_OK) return st;
@@ -430,10 +377,10 @@ lqg_status lqg_search_batch_device(lqg_index* h, const float* d_queries, uint64_
         ef_max = h->ef;
     }
     if (ef_class(ef_max) == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported");
-    lqg_status st = ensure_device(h);
+    lqg_status st = lqg::ensure_device(h);
     if (st != LQG_OK) return st;
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->stream;
-    return run_batch(h, d_queries, nq, d_ef, d_k, ef_max, kmax, order, d_results, d_dists, d_stats,
+    return lqg::run_batch(h, d_queries, nq, d_ef, d_k, ef_max, kmax, order, d_results, d_dists, d_stats,
                      s);
 }
 
@@ -458,14 +405,19 @@ lqg_status lqg_get_graph(const lqg_index* h, uint32_t* neighbors) {
 lqg_status lqg_build_index(lqg_index* h, const float* data, uint64_t num_points,
                            uint64_t dimension, uint32_t ef_build, uint32_t num_iter,
                            uint32_t num_thread) {
-    (void)data;
-    (void)ef_build;
-    (void)num_iter;
-    (void)num_thread;
-    if (!h) return fail(LQG_ERR_ARG, "null index");
+    (void)num_thread;  // the build runs on the device
+    if (!h || !data) return fail(LQG_ERR_ARG, "null argument");
     if (num_points != h->n || dimension != h->dim)
         return fail(LQG_ERR_ARG, "The shape of data is different with initialization!");
-    return fail(LQG_ERR_UNSUPPORTED, "build_index is not available yet");
+    if (num_iter <= 1) return fail(LQG_ERR_ARG, "The number of iter for building qg should >= 3");
+    if (ef_build == 0) return fail(LQG_ERR_ARG, "EF must be positive");
+    if (ef_class(ef_build) == 0)
+        return fail(LQG_ERR_UNSUPPORTED, "build EF above 1024 is not supported");
+    if (h->deg > 64) return fail(LQG_ERR_UNSUPPORTED, "build_index supports degree_bound <= 64");
+    if (h->n <= h->deg) return fail(LQG_ERR_ARG, "num_elements must exceed degree_bound");
+    if (check_queries(data, num_points * dimension) != LQG_OK)
+        return fail(LQG_ERR_ARG, "data contains a non-finite value");
+    return lqg::build_index(h, data, ef_build, num_iter);
 }
 
 }  // extern "C"

@@ -1,0 +1,104 @@
+// lqg_index.h -- the index handle behind the C ABI, shared by the host files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/liteqg.h"
+#include "lqg_internal.h"
+
+namespace lqg {
+
+lqg_status fail(lqg_status s, const std::string& msg);
+lqg_status hip_fail(hipError_t e, const char* what);
+const char* last_error();
+
+#define LQG_HIP_TRY(expr)                                     \
+    do {                                                      \
+        hipError_t e_ = (expr);                               \
+        if (e_ != hipSuccess) return ::lqg::hip_fail(e_, #expr); \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;  // elements
+    hipError_t reserve(size_t n) {
+        if (n <= cap) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) cap = n;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Smallest pool-capacity class that holds ef, 0 if none does.
+int ef_class(uint32_t ef);
+
+}  // namespace lqg
+
+struct lqg_index {
+    uint64_t n = 0, dim = 0, deg = 0;
+    uint32_t ep = 0;
+    uint32_t ef = 0;
+    bool ef_set = false;
+    bool have_data = false;  // loaded or built
+    lqg_order order = LQG_ORDER_REFERENCE;
+
+    // host copy: exactly the file's content, split by section
+    std::vector<float> raw;      // n x dim
+    std::vector<uint32_t> nbr;   // n x deg
+    std::vector<uint8_t> qdata;  // n x dim
+    std::vector<float> sq8;      // dim x {min, scale}
+
+    // device copy
+    bool on_device = false;
+    hipStream_t stream = nullptr;
+    lqg::DevBuf<uint8_t> d_qdata;
+    lqg::DevBuf<uint32_t> d_nbr;
+    lqg::DevBuf<float> d_min, d_scale;
+    // per-call scratch (grow only)
+    lqg::DevBuf<float> d_q;
+    lqg::DevBuf<uint32_t> d_ids, d_ef, d_k, d_stats, d_counter;
+    lqg::DevBuf<float> d_dist;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    lqg::DevIndex dev() const {
+        lqg::DevIndex ix;
+        ix.qdata = d_qdata.p;
+        ix.nbr = d_nbr.p;
+        ix.sq8_min = d_min.p;
+        ix.sq8_scale = d_scale.p;
+        ix.n = static_cast<uint32_t>(n);
+        ix.dim = static_cast<uint32_t>(dim);
+        ix.deg = static_cast<uint32_t>(deg);
+        ix.ep = ep;
+        return ix;
+    }
+};
+
+namespace lqg {
+
+// Checks the device, creates the stream and uploads the host copy (once).
+lqg_status ensure_device(lqg_index* h);
+
+// Enqueues one search launch over device arrays.
+lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
+                     const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, lqg_order order,
+                     uint32_t* d_results, float* d_dists, uint32_t* d_stats, hipStream_t s);
+
+// Device graph build for lqg_build_index.
+lqg_status build_index(lqg_index* h, const float* data, uint32_t ef_build, uint32_t num_iter);
+
+}  // namespace lqg

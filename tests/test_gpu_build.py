@@ -1,0 +1,112 @@
+"""Device graph builder (Index.build_index): file format, graph validity, recall.
+
+The reference's builder is not reproducible (std::random_device), so the bar
+is structural validity of every row, byte-exact save/load, and recall at
+least that of the reference-built graph on the same data.
+"""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def built_index(base, deg=32, ef=100, iters=3):
+    from liteqg_amd.symphonyqg import Index
+
+    n, d = base.shape
+    ix = Index("QG", "L2", n, d, deg)
+    ix.build_index(base, ef, num_iter=iters)
+    return ix
+
+
+def check_rows(g, n):
+    assert (g < n).all()
+    assert (g != np.arange(n, dtype=np.uint32)[:, None]).all(), "self loop"
+    s = np.sort(g, axis=1)
+    assert (s[:, 1:] != s[:, :-1]).all(), "repeated neighbour in a row"
+
+
+def recall(oracle_mod, ix, base, Q, ef, k=10):
+    gt, _ = oracle_mod.bruteforce_fp64(base, Q, k)
+    ids = ix.search_batch(Q, k, ef=ef, order="nearest")
+    return oracle_mod.recall_at_k(ids, gt, k)
+
+
+def test_build_small_uniform(oracle_mod, tmp_path):
+    rng = np.random.default_rng(0)
+    base = rng.random((5000, 32), dtype=np.float32)
+    ix = built_index(base, ef=100)
+    g = ix.graph()
+    check_rows(g, 5000)
+    info = ix.info()
+    c = base.mean(0)
+    assert info["entry_point"] == int(np.argmin(((base - c) ** 2).sum(1)))
+    Q = rng.random((200, 32), dtype=np.float32)
+    assert recall(oracle_mod, ix, base, Q, 64) > 0.95
+    # save -> load -> identical file and identical search
+    p = tmp_path / "b.index"
+    ix.save(str(p))
+    from liteqg_amd.symphonyqg import Index
+
+    ix2 = Index("QG", "L2", 5000, 32, 32)
+    ix2.load(str(p))
+    assert np.array_equal(ix2.graph(), g)
+    assert np.array_equal(ix2.search_batch(Q, 10, ef=64), ix.search_batch(Q, 10, ef=64))
+    # the file's SQ8 section is what the reference's quantiser writes
+    arr = oracle_mod.read_index(str(p), 5000, 32, 32)
+    codes, params = oracle_mod.sq8_quantize(base)
+    assert np.array_equal(arr.qdata, codes)
+    assert np.array_equal(arr.params.view(np.uint32), params.view(np.uint32))
+    assert np.array_equal(arr.raw, base)
+
+
+def test_build_is_repeatable():
+    rng = np.random.default_rng(1)
+    base = rng.random((3000, 16), dtype=np.float32)
+    a = built_index(base, ef=64).graph()
+    b = built_index(base, ef=64).graph()
+    assert np.array_equal(np.sort(a, 1), np.sort(b, 1))
+
+
+@pytest.mark.parametrize("name,n,d,deg", [("g_uniform_n2000_d64", 2000, 64, 32),
+                                          ("g_intmix_n1200_d128", 1200, 128, 32),
+                                          ("g_unitmix_n1000_d96_deg64", 1000, 96, 64)])
+def test_recall_not_below_reference_build(oracle_mod, name, n, d, deg):
+    """Same data as a reference-built fixture: our graph's recall is not lower."""
+    ref = oracle_mod.read_index(f"{GOLDEN}/{name}.index", n, d, deg)
+    base = ref.raw.copy()
+    z = np.load(f"{GOLDEN}/{name}.npz")
+    Q = z["queries"]
+    ix = built_index(base, deg=deg, ef=200)
+    check_rows(ix.graph(), n)
+    gt, _ = oracle_mod.bruteforce_fp64(base, Q, 10)
+    for ef in (10, 20, 50):
+        mine = oracle_mod.recall_at_k(ix.search_batch(Q, 10, ef=ef, order="nearest"), gt, 10)
+        theirs_ids, _, _ = oracle_mod.search_canon_batch(ref, Q, ef, 10, "nearest")
+        theirs = oracle_mod.recall_at_k(theirs_ids, gt, 10)
+        assert mine >= theirs - 0.02, (ef, mine, theirs)
+
+
+def test_build_clustered_int_data(oracle_mod):
+    """Well separated clusters (the 1M x 128 workload's process, shrunk)."""
+    from liteqg_amd import datasets
+
+    base, Q = datasets.make("int_mix", 20000, 128, 200, seed=3, centres=64, sigma=20.0)
+    ix = built_index(base, ef=200)
+    check_rows(ix.graph(), 20000)
+    assert recall(oracle_mod, ix, base, Q, 100) > 0.95
+
+
+def test_build_argument_errors():
+    from liteqg_amd.symphonyqg import Index, LqgError
+
+    ix = Index("QG", "L2", 100, 8, 32)
+    data = np.zeros((100, 8), np.float32)
+    with pytest.raises(LqgError):
+        ix.build_index(data, 50, num_iter=1)
+    with pytest.raises(LqgError):
+        Index("QG", "L2", 20, 8, 32).build_index(np.zeros((20, 8), np.float32), 50)
+    with pytest.raises(LqgError):
+        Index("QG", "L2", 200, 8, 96).build_index(np.zeros((200, 8), np.float32), 50)
