@@ -82,7 +82,10 @@ class DeviceBatch:
         self.q = torch.from_numpy(Q).cuda()
         self.ids = torch.empty((self.nq, k), dtype=torch.int32, device="cuda")
         self.stats = torch.empty((self.nq, 2), dtype=torch.int32, device="cuda")
-        self.stream = torch.cuda.current_stream()
+        # a stream of our own (default priority): its handle is never NULL, so
+        # the kernel runs on exactly the stream the timing events are recorded on
+        self.stream = torch.cuda.Stream()
+        torch.cuda.synchronize()
 
     def launch(self, ef, with_stats=False):
         self.check(self.L.lqg_search_batch_device(
@@ -213,6 +216,9 @@ def main():
     if chosen is None:
         chosen = sweep[-1]["ef"]
     rec = next(s["recall"] for s in sweep if s["ef"] == chosen)
+    t = time.perf_counter()
+    tuned = ix.autotune(Q, chosen, k)
+    log(f"autotune ef={chosen}: {tuned} ({time.perf_counter() - t:.1f}s)")
 
     # timed region: K launches over the resident queries
     ms_step, wall = time_launches(db, chosen, args.steps, args.warmup)
@@ -266,7 +272,7 @@ def main():
                         f"build EF {ef_build} x3 iterations",
             "ef": chosen, "k": k, "nq": args.nq, "path": "SQ8 beam search, order=nearest",
             "recall_at_10": rec, "ef_sweep": sweep, "build_s": round(build_s, 1),
-            "seed": args.seed,
+            "seed": args.seed, "launch": tuned,
         },
         "plain_gpu": {"qps": round(plain_qps, 1), "p50_ms": round(float(np.median(lat)), 4),
                       "p99_ms": round(float(np.percentile(lat, 99)), 4), "queries": len(lat)},

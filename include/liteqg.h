@@ -104,6 +104,29 @@ lqg_status lqg_search_batch_device(lqg_index* index, const float* d_queries, uin
                                    uint32_t kmax, lqg_order order, uint32_t* d_results,
                                    float* d_dists, uint32_t* d_stats, void* stream);
 
+/* Launch settings of the search kernel for one pool-capacity class
+ * (ef <= 64, 128, 256, 512, 1024).  No counterpart in the reference (CPU
+ * code has no launch geometry). */
+typedef struct {
+    int32_t waves_per_group; /* wavefronts (queries in flight) per workgroup, 1..4      */
+    int32_t cache_bits;      /* log2 entries of the per-wave visited cache, 4..14       */
+    int32_t occupancy;       /* 8: kernel capped at 64 VGPRs (8 waves/SIMD); 0: uncapped */
+    int32_t groups;          /* workgroups of the persistent grid; 0 = fill the device  */
+} lqg_launch;
+
+/* Sets the launch settings used for every ef in the class of `ef` (ef = 0:
+ * all classes); cfg = NULL restores the defaults. */
+lqg_status lqg_set_launch(lqg_index* index, uint32_t ef, const lqg_launch* cfg);
+
+/* Reads the launch settings in force for the class of `ef`. */
+lqg_status lqg_get_launch(const lqg_index* index, uint32_t ef, lqg_launch* cfg);
+
+/* Autotuner: times each candidate launch setting on nq queries (host
+ * memory) at (ef, k), keeps the fastest for ef's class and returns it in
+ * best (optional).  Results are unaffected by the setting; only speed is. */
+lqg_status lqg_autotune(lqg_index* index, const float* queries, uint64_t nq, uint32_t ef,
+                        uint32_t k, lqg_launch* best);
+
 /* Read-only views of the loaded index (host copies). */
 lqg_status lqg_info(const lqg_index* index, uint64_t* num_elements, uint64_t* dimension,
                     uint64_t* degree_bound, uint32_t* entry_point, uint32_t* ef);

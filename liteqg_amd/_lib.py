@@ -23,6 +23,11 @@ class LqgError(RuntimeError):
         self.status = status
 
 
+class Launch(C.Structure):
+    _fields_ = [("waves_per_group", C.c_int32), ("cache_bits", C.c_int32),
+                ("occupancy", C.c_int32), ("groups", C.c_int32)]
+
+
 class Stats(C.Structure):
     _fields_ = [("queries", C.c_uint64), ("expansions", C.c_uint64),
                 ("evaluations", C.c_uint64), ("device_ms", C.c_double)]
@@ -44,6 +49,9 @@ _SIGS = {
     "lqg_search_batch": [_P, _P, C.c_uint64, _P, _P, C.c_uint32, C.c_int, _P, _P, _P],
     "lqg_search_batch_device": [_P, _P, C.c_uint64, _P, _P, C.c_uint32, C.c_uint32, C.c_int, _P,
                                 _P, _P, _P],
+    "lqg_set_launch": [_P, C.c_uint32, _P],
+    "lqg_get_launch": [_P, C.c_uint32, _P],
+    "lqg_autotune": [_P, _P, C.c_uint64, C.c_uint32, C.c_uint32, _P],
     "lqg_info": [_P, _P, _P, _P, _P, _P],
     "lqg_get_graph": [_P, _P],
 }

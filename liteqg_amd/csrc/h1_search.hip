@@ -5,12 +5,19 @@
 //
 // One wavefront walks one query.  Keys are (distance, id) packed into 64 bits
 //   key = float_bits(d) << 32 | id << 1 | expanded
-// so that comparing key >> 1 orders by distance and then id (distances are
-// squared L2 >= +0, whose float bits are monotone).  Under this total order
-// the reference's two heaps reduce to one sorted pool of capacity ef whose
+// so that comparing keys orders by distance and then id (distances are squared
+// L2 >= +0, whose float bits are monotone).  Under this total order the
+// reference's two heaps reduce to one sorted pool of capacity ef whose
 // smallest unexpanded entry is expanded until none is left (an entry pushed
 // out of res_pool always compares above the current lowerBound, and the
 // reference stops at the first such top, qg.hpp:316-317).
+//
+// The pool is held in registers, lane-major: logical entry i = s*64 + lane is
+// P[s] of that lane (EF_CAP/64 slots per lane), unused entries ~0.  Candidates
+// of one expansion are admitted in one uniform pass that computes, with
+// readlane broadcasts and ballots, each candidate's position, duplicate test
+// and rank, and each pool entry's shift; one scatter/gather through LDS then
+// applies the permutation.
 //
 // Visited set: a direct-mapped cache of ids in LDS (lossy).  A cache hit is a
 // node evaluated before, so skipping it is what the reference does.  A miss
@@ -19,13 +26,17 @@
 // either still in the pool (same key -> found) or compares above the pool's
 // current maximum (it was rejected or pushed out when the maximum was larger,
 // and the maximum never grows).  The result is therefore exactly that of an
-// exact visited set; only the number of evaluations can grow.
+// exact visited set; only the number of evaluations can grow.  Repeated ids
+// inside one row are masked out of the device copy of the neighbour table.
 //
 // Distances follow the reference's AVX-512 rounding order exactly: partial
 // sum p (0..15) accumulates dims p, p+16, p+32, ... with one FMA each, then
-// pairs (p, p+8), (p, p+4), (p, p+2), (0, 1) are added.  A quad of lanes
-// evaluates one candidate: lane t holds partials 4t..4t+3, i.e. the dword at
-// byte 16i+4t of each 16-byte chunk i of the SQ8 row.
+// pairs (p, p+8), (p, p+4), (p, p+2), (0, 1) are added.  Neighbour rows are
+// walked in chunks of 32 ids; a group of L lanes (L = 4 or 2, chosen so the
+// fresh candidates of one chunk fit one pass) evaluates one candidate: lane t
+// of the group holds partials t*16/L .. (t+1)*16/L - 1, i.e. 16/L consecutive
+// bytes of each 16-byte chunk of the SQ8 row.  The candidate's key ends up in
+// the group's first lane.
 #include <math.h>
 
 #include "lqg_internal.h"
@@ -47,7 +58,6 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 __device__ __forceinline__ uint64_t make_key(float d, uint32_t id) {
     return (static_cast<uint64_t>(__float_as_uint(d)) << 32) | (static_cast<uint64_t>(id) << 1);
 }
-__device__ __forceinline__ uint64_t kcmp(uint64_t k) { return k >> 1; }
 __device__ __forceinline__ uint32_t key_id(uint64_t k) {
     return static_cast<uint32_t>(k >> 1) & 0x7FFFFFFFu;
 }
@@ -55,162 +65,191 @@ __device__ __forceinline__ float key_dist(uint64_t k) {
     return __uint_as_float(static_cast<uint32_t>(k >> 32));
 }
 
-// Number of entries of the sorted LDS array a[0..n) whose key is below x.
-__device__ __forceinline__ uint32_t lower_bound(const uint64_t* a, uint32_t n, uint64_t x) {
-    const uint64_t xk = kcmp(x);
-    uint32_t lo = 0;
-    while (n > 0) {
-        const uint32_t h = n >> 1;
-        if (kcmp(a[lo + h]) < xk) {
-            lo += h + 1;
-            n -= h + 1;
-        } else {
-            n = h;
-        }
-    }
-    return lo;
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v)), l);
+    const uint32_t hi =
+        __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v >> 32)), l);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
-__device__ __forceinline__ void acc4(uint32_t w, const float4& qq, const float4& mn,
-                                     const float4& sc, float& s0, float& s1, float& s2,
-                                     float& s3) {
-    // x = min + q * scale, contracted to one FMA as the reference build does
-    float x, t;
-    x = fmaf(static_cast<float>(w & 0xFFu), sc.x, mn.x);
-    t = qq.x - x;
-    s0 = fmaf(t, t, s0);
-    x = fmaf(static_cast<float>((w >> 8) & 0xFFu), sc.y, mn.y);
-    t = qq.y - x;
-    s1 = fmaf(t, t, s1);
-    x = fmaf(static_cast<float>((w >> 16) & 0xFFu), sc.z, mn.z);
-    t = qq.z - x;
-    s2 = fmaf(t, t, s2);
-    x = fmaf(static_cast<float>(w >> 24), sc.w, mn.w);
-    t = qq.w - x;
-    s3 = fmaf(t, t, s3);
+// One SQ8 byte into partial s: x = min + q*scale (one FMA, as the reference
+// build contracts it), t = query - x, s += t*t (one FMA).
+__device__ __forceinline__ void acc1(uint32_t byte, float qq, float mn, float sc, float& s) {
+    const float x = fmaf(static_cast<float>(byte), sc, mn);
+    const float t = qq - x;
+    s = fmaf(t, t, s);
 }
 
-// Keys of the m candidates bids[0..m) into bkeys[0..m).  All 64 lanes call it.
-__device__ void eval_batch(const DevIndex& ix, const float* qv, const float* pmin,
-                           const float* pscale, const uint32_t* bids, uint64_t* bkeys,
-                           uint32_t m, uint32_t lane) {
-    const uint32_t t = lane & 3u;
+// Squared L2 of the query to candidate `id` by a group of L lanes (t = lane
+// index in the group).  Returns the full sum on every lane of the group.
+template <int L>
+__device__ __forceinline__ float group_l2(const DevIndex& ix, const float* qv, const float* pmin,
+                                         const float* pscale, uint32_t id, bool act,
+                                         uint32_t t) {
+    constexpr int PL = 16 / L;  // partials (and bytes per chunk) per lane
+    float s[PL];
+#pragma unroll
+    for (int p = 0; p < PL; ++p) s[p] = 0.f;
     const uint32_t nchunk = ix.dim >> 4;
-    const uint32_t tail0 = nchunk << 4;
-    for (uint32_t base = 0; base < m; base += 16) {
-        const uint32_t c = base + (lane >> 2);
-        const bool act = c < m;
-        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-        uint32_t id = 0;
-        const uint8_t* row = nullptr;
-        if (act) {
-            id = bids[c];
-            row = ix.qdata + static_cast<size_t>(id) * ix.dim;
-            if ((ix.dim & 3u) == 0) {
-                const uint32_t* rw = reinterpret_cast<const uint32_t*>(row) + t;
-#pragma unroll 4
-                for (uint32_t i = 0; i < nchunk; ++i) {
-                    const uint32_t w = rw[4 * i];
-                    const uint32_t b = 16 * i + 4 * t;
-                    acc4(w, *reinterpret_cast<const float4*>(qv + b),
-                         *reinterpret_cast<const float4*>(pmin + b),
-                         *reinterpret_cast<const float4*>(pscale + b), s0, s1, s2, s3);
+    const uint8_t* row = ix.qdata + static_cast<size_t>(id) * ix.dim;
+    if (act) {
+        if ((ix.dim & 15u) == 0) {
+            // 16-byte aligned rows: PL bytes of each chunk in one load.  All loads
+            // of up to 8 chunks (128 dims) are issued before any arithmetic, so a
+            // row of <= 128 bytes costs one memory round trip.
+            constexpr int CB = 8;
+            constexpr int W = PL / 4;  // dwords per chunk per lane
+            for (uint32_t base = 0; base < nchunk; base += CB) {
+                uint32_t wv[CB][W];
+#pragma unroll
+                for (int c = 0; c < CB; ++c) {
+                    if (base + c < nchunk) {
+                        const uint8_t* src = row + 16 * (base + c) + PL * t;
+                        if constexpr (W == 2) {
+                            const uint2 v = *reinterpret_cast<const uint2*>(src);
+                            wv[c][0] = v.x;
+                            wv[c][1] = v.y;
+                        } else {
+                            wv[c][0] = *reinterpret_cast<const uint32_t*>(src);
+                        }
+                    }
                 }
-            } else {
-                for (uint32_t i = 0; i < nchunk; ++i) {
-                    const uint32_t b = 16 * i + 4 * t;
-                    const uint32_t w = static_cast<uint32_t>(row[b]) |
-                                       (static_cast<uint32_t>(row[b + 1]) << 8) |
-                                       (static_cast<uint32_t>(row[b + 2]) << 16) |
-                                       (static_cast<uint32_t>(row[b + 3]) << 24);
-                    const float4 qq = make_float4(qv[b], qv[b + 1], qv[b + 2], qv[b + 3]);
-                    const float4 mn =
-                        make_float4(pmin[b], pmin[b + 1], pmin[b + 2], pmin[b + 3]);
-                    const float4 sc =
-                        make_float4(pscale[b], pscale[b + 1], pscale[b + 2], pscale[b + 3]);
-                    acc4(w, qq, mn, sc, s0, s1, s2, s3);
+#pragma unroll
+                for (int c = 0; c < CB; ++c) {
+                    if (base + c < nchunk) {
+                        const uint32_t b = 16 * (base + c) + PL * t;
+#pragma unroll
+                        for (int v = 0; v < W; ++v) {
+                            const float4 qq = *reinterpret_cast<const float4*>(qv + b + 4 * v);
+                            const float4 mn = *reinterpret_cast<const float4*>(pmin + b + 4 * v);
+                            const float4 sc = *reinterpret_cast<const float4*>(pscale + b + 4 * v);
+                            const uint32_t x = wv[c][v];
+                            acc1(x & 0xFFu, qq.x, mn.x, sc.x, s[4 * v + 0]);
+                            acc1((x >> 8) & 0xFFu, qq.y, mn.y, sc.y, s[4 * v + 1]);
+                            acc1((x >> 16) & 0xFFu, qq.z, mn.z, sc.z, s[4 * v + 2]);
+                            acc1(x >> 24, qq.w, mn.w, sc.w, s[4 * v + 3]);
+                        }
+                    }
                 }
             }
-        }
-        // (p, p+8): lanes t and t^2;  (p, p+4): lanes t and t^1
-        s0 += __shfl_xor(s0, 2);
-        s1 += __shfl_xor(s1, 2);
-        s2 += __shfl_xor(s2, 2);
-        s3 += __shfl_xor(s3, 2);
-        s0 += __shfl_xor(s0, 1);
-        s1 += __shfl_xor(s1, 1);
-        s2 += __shfl_xor(s2, 1);
-        s3 += __shfl_xor(s3, 1);
-        float r = (s0 + s2) + (s1 + s3);  // (0,2),(1,3) then (0,1)
-        if (act) {
-            for (uint32_t d = tail0; d < ix.dim; ++d) {  // l2.hpp:31-34
-                const float x = fmaf(static_cast<float>(row[d]), pscale[d], pmin[d]);
-                const float tt = qv[d] - x;
-                r = fmaf(tt, tt, r);
-            }
-            if (t == 0) bkeys[c] = make_key(r, id);
+        } else {
+            for (uint32_t i = 0; i < nchunk; ++i)
+#pragma unroll
+                for (int p = 0; p < PL; ++p) {
+                    const uint32_t d = 16 * i + PL * t + p;
+                    acc1(row[d], qv[d], pmin[d], pscale[d], s[p]);
+                }
         }
     }
+    float r;
+    if constexpr (L == 4) {
+        // lane t holds partials 4t..4t+3: (p, p+8) across t^2, (p, p+4) across t^1
+#pragma unroll
+        for (int p = 0; p < 4; ++p) s[p] += __shfl_xor(s[p], 2);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) s[p] += __shfl_xor(s[p], 1);
+        r = (s[0] + s[2]) + (s[1] + s[3]);
+    } else {
+        static_assert(L == 2, "groups of 4 or 2 lanes");
+        // lane t holds partials 8t..8t+7: (p, p+8) across t^1, then in-lane
+#pragma unroll
+        for (int p = 0; p < 8; ++p) s[p] += __shfl_xor(s[p], 1);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) s[p] += s[p + 4];
+        r = (s[0] + s[2]) + (s[1] + s[3]);
+    }
+    if (act) {
+        for (uint32_t d = nchunk << 4; d < ix.dim; ++d) acc1(row[d], qv[d], pmin[d], pscale[d], r);
+    }
+    return r;
+}
+
+// Keys of the m fresh candidates bids[0..m) (m <= 32) in one pass: candidate
+// c's key is returned on lane c*L (L = 4 for m <= 16, else 2); other lanes
+// return ~0.  All 64 lanes call it.
+__device__ __forceinline__ uint64_t eval_fresh(const DevIndex& ix, const float* qv,
+                                               const float* pmin, const float* pscale,
+                                               const uint32_t* bids, uint32_t m,
+                                               uint32_t lane) {
+    float r;
+    uint32_t c, L;
+    if (m <= 16) {
+        L = 4;
+        c = lane >> 2;
+        const bool act = c < m;
+        r = group_l2<4>(ix, qv, pmin, pscale, act ? bids[c] : 0u, act, lane & 3u);
+    } else {
+        L = 2;
+        c = lane >> 1;
+        const bool act = c < m;
+        r = group_l2<2>(ix, qv, pmin, pscale, act ? bids[c] : 0u, act, lane & 1u);
+    }
+    const bool leader = c < m && (lane & (L - 1)) == 0;
+    return leader ? make_key(r, bids[c]) : ~0ull;
 }
 
 struct WaveLds {
-    float* qv;
-    uint64_t* pool;
-    uint64_t* bkeys;
-    uint64_t* bsorted;
-    uint32_t* bids;
-    uint32_t* cache;
+    float* qv;        // query (dim)
+    uint64_t* stage;  // pool permutation staging (EF_CAP)
+    uint32_t* bids;   // fresh candidate ids (64)
+    uint32_t* cache;  // visited cache (1 << cache_bits)
 };
 
 template <int EF_CAP>
 __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
                            const float* pmin, const float* pscale, const WaveLds& w,
                            uint32_t cache_bits, uint32_t lane) {
-    // clamps keep LDS and output writes in bounds whatever the arrays hold
-    const uint32_t ef = min(job.ef ? job.ef[q] : job.ef_all, static_cast<uint32_t>(EF_CAP));
+    constexpr int S = EF_CAP / kWave;
+    // clamps keep register, LDS and output indexing in bounds whatever the arrays hold
+    const uint32_t ef =
+        max(1u, min(job.ef ? job.ef[q] : job.ef_all, static_cast<uint32_t>(EF_CAP)));
     const uint32_t kq = min(job.k ? job.k[q] : job.k_all, job.kstride);
     const float* qsrc = job.queries + static_cast<size_t>(q) * ix.dim;
     for (uint32_t i = lane; i < ix.dim; i += kWave) w.qv[i] = qsrc[i];
     const uint32_t csize = 1u << cache_bits;
     for (uint32_t i = lane; i < csize; i += kWave) w.cache[i] = kInvalid;
-    if (lane == 0) w.bids[0] = ix.ep;
-    wave_sync();
-
-    // entry point: qg.hpp:305-311
-    eval_batch(ix, w.qv, pmin, pscale, w.bids, w.bkeys, 1, lane);
-    wave_sync();
     if (lane == 0) {
-        w.pool[0] = w.bkeys[0];
+        w.bids[0] = ix.ep;
         w.cache[(ix.ep * 2654435761u) >> (32 - cache_bits)] = ix.ep;
     }
     wave_sync();
-    uint32_t size = 1, cursor = 0, n_exp = 0, n_eval = 1;
+
+    // entry point: qg.hpp:305-311
+    uint64_t P[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) P[s] = ~0ull;
+    {
+        const uint64_t k0 = readlane64(eval_fresh(ix, w.qv, pmin, pscale, w.bids, 1, lane), 0);
+        if (lane == 0) P[0] = k0;
+    }
+    uint32_t size = 1, n_exp = 0, n_eval = 1;
 
     for (;;) {
         // pop the smallest unexpanded entry (qg.hpp:314-318)
-        uint32_t c = size;
-        for (uint32_t b = cursor; b < size; b += kWave) {
-            const uint32_t i = b + lane;
-            const bool un = (i < size) && !(w.pool[i] & 1ull);
-            const uint64_t mk = __ballot(un);
-            if (mk) {
-                c = b + static_cast<uint32_t>(__builtin_ctzll(mk));
-                break;
+        uint64_t kc = 0;
+        bool found = false;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (!found) {
+                const uint32_t i = static_cast<uint32_t>(s * kWave) + lane;
+                const uint64_t m = __ballot(i < size && !(P[s] & 1ull));
+                if (m) {
+                    const int l = static_cast<int>(__builtin_ctzll(m));
+                    kc = readlane64(P[s], l);
+                    if (static_cast<int>(lane) == l) P[s] |= 1ull;
+                    found = true;
+                }
             }
         }
-        if (c >= size) break;
-        const uint64_t kc = w.pool[c];
-        wave_sync();
-        if (lane == 0) w.pool[c] = kc | 1ull;
-        cursor = c + 1;
+        if (!found) break;
         ++n_exp;
         const uint32_t* nrow = ix.nbr + static_cast<size_t>(key_id(kc)) * ix.deg;
 
-        for (uint32_t cb = 0; cb < ix.deg; cb += kWave) {
-            // neighbours not seen before (qg.hpp:320-324)
-            uint32_t nid = kInvalid;
+        for (uint32_t cb = 0; cb < ix.deg; cb += 32) {
+            // neighbours not seen before (qg.hpp:320-324), 32 ids at a time
             bool fresh = false;
-            if (cb + lane < ix.deg) {
+            uint32_t nid = kInvalid;
+            if (lane < 32 && cb + lane < ix.deg) {
                 nid = nrow[cb + lane];
                 if (nid < ix.n) {
                     const uint32_t slot = (nid * 2654435761u) >> (32 - cache_bits);
@@ -219,79 +258,70 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
             }
             const uint64_t fm = __ballot(fresh);
             if (fm == 0) continue;
-            const uint32_t fp = lanes_below(fm);
-            wave_sync();
-            if (fresh) w.bids[fp] = nid;
-            wave_sync();
-            // repeated ids in one list: the first occurrence is the one evaluated
-            bool keep = fresh;
-            if (fresh) {
-                for (uint32_t i = 0; i < fp; ++i) {
-                    if (w.bids[i] == nid) {
-                        keep = false;
-                        break;
-                    }
-                }
-            }
-            const uint64_t km = __ballot(keep);
-            const uint32_t m = static_cast<uint32_t>(__popcll(km));
-            if (km != fm) {
-                wave_sync();
-                if (keep) w.bids[lanes_below(km)] = nid;
-            }
+            const uint32_t m = static_cast<uint32_t>(__popcll(fm));
+            if (fresh) w.bids[lanes_below(fm)] = nid;
             wave_sync();
             n_eval += m;
-            eval_batch(ix, w.qv, pmin, pscale, w.bids, w.bkeys, m, lane);
-            wave_sync();
+            const uint64_t key = eval_fresh(ix, w.qv, pmin, pscale, w.bids, m, lane);
 
-            // admission (qg.hpp:329-335): below the pool maximum, or pool not full;
-            // and not already in the pool
-            bool cand = lane < m;
-            const uint64_t key = cand ? w.bkeys[lane] : ~0ull;
-            const uint64_t last = w.pool[size - 1];
-            if (cand && size >= ef && !(kcmp(key) < kcmp(last))) cand = false;
-            uint32_t lb = 0;
-            if (cand) {
-                lb = lower_bound(w.pool, size, key);
-                if (lb < size && kcmp(w.pool[lb]) == kcmp(key)) cand = false;
-            }
-            const uint64_t cm = __ballot(cand);
-            if (cm == 0) continue;
-            const uint32_t m3 = static_cast<uint32_t>(__popcll(cm));
-            wave_sync();
-            if (cand) w.bkeys[lanes_below(cm)] = key;
-            wave_sync();
-            uint32_t rank = 0;
-            if (cand) {
-                for (uint32_t i = 0; i < m3; ++i) rank += (kcmp(w.bkeys[i]) < kcmp(key)) ? 1u : 0u;
-                w.bsorted[rank] = key;
-            }
-            const uint32_t npos = lb + rank;
-            const uint64_t r0 = __ballot(cand && rank == 0);
-            const uint32_t first_new = __shfl(npos, static_cast<int>(__builtin_ctzll(r0)));
-            wave_sync();
-
-            // merge: an entry at i moves to i + (new keys below it); keep ef
-            const uint32_t nsize = min(size + m3, ef);
-            uint64_t pv[EF_CAP / kWave];
-            uint32_t pn[EF_CAP / kWave];
+            // admission (qg.hpp:329-335): pool not full, or below its maximum
+            uint64_t last = 0;
 #pragma unroll
-            for (int j = 0; j < EF_CAP / kWave; ++j) {
-                const uint32_t i = static_cast<uint32_t>(j * kWave) + lane;
-                pn[j] = kInvalid;
-                pv[j] = 0;
-                if (i >= first_new && i < size) {
-                    pv[j] = w.pool[i];
-                    pn[j] = i + lower_bound(w.bsorted, m3, pv[j]);
+            for (int s = 0; s < S; ++s)
+                if (static_cast<uint32_t>(s) == (size - 1) / kWave)
+                    last = readlane64(P[s], static_cast<int>((size - 1) % kWave));
+            bool cand = key != ~0ull && (size < ef || key < last);
+            uint64_t mm = __ballot(cand);
+            if (mm == 0) continue;
+            // one pass over the candidates: position in the pool, duplicate test,
+            // rank among candidates, shift of every pool entry.  Candidate keys
+            // have bit 0 clear, so plain comparisons with pool keys order by
+            // (distance, id) except for a candidate equal to a pool entry, which
+            // is detected first and dropped.
+            uint32_t lb = 0, rank = 0;
+            uint32_t shift[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) shift[s] = 0;
+            bool drop = false;
+            while (mm) {
+                const int j = static_cast<int>(__builtin_ctzll(mm));
+                mm &= mm - 1;
+                const uint64_t kj = readlane64(key, j);
+                uint32_t below = 0;
+                bool eq = false;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    below += static_cast<uint32_t>(__popcll(__ballot(P[s] < kj)));
+                    eq |= __ballot(((P[s] ^ kj) >> 1) == 0) != 0;
                 }
+                if (eq) {  // already in the pool (evaluated before, forgotten by the cache)
+                    if (static_cast<int>(lane) == j) drop = true;
+                    continue;
+                }
+                if (static_cast<int>(lane) == j) lb = below;
+                rank += kj < key ? 1u : 0u;
+#pragma unroll
+                for (int s = 0; s < S; ++s) shift[s] += kj < P[s] ? 1u : 0u;
             }
+            cand = cand && !drop;
+            const uint32_t m3 = static_cast<uint32_t>(__popcll(__ballot(cand)));
+            if (m3 == 0) continue;
+            // merge: scatter to new positions, keep ef (res_pool.pop, qg.hpp:332-333)
+            const uint32_t nsize = min(size + m3, ef);
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const uint32_t i = static_cast<uint32_t>(s * kWave) + lane;
+                const uint32_t np = i + shift[s];
+                if (i < size && np < nsize) w.stage[np] = P[s];
+            }
+            if (cand && lb + rank < nsize) w.stage[lb + rank] = key;
             wave_sync();
 #pragma unroll
-            for (int j = 0; j < EF_CAP / kWave; ++j)
-                if (pn[j] < nsize) w.pool[pn[j]] = pv[j];
-            if (cand && npos < nsize) w.pool[npos] = key;
+            for (int s = 0; s < S; ++s) {
+                const uint32_t i = static_cast<uint32_t>(s * kWave) + lane;
+                P[s] = i < nsize ? w.stage[i] : ~0ull;
+            }
             size = nsize;
-            cursor = min(cursor, first_new);
             wave_sync();
         }
     }
@@ -301,23 +331,21 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
     uint32_t* oi = job.out_ids + static_cast<size_t>(q) * job.kstride;
     float* od = job.out_d ? job.out_d + static_cast<size_t>(q) * job.kstride : nullptr;
     for (uint32_t j = lane; j < job.kstride; j += kWave) {
-        uint32_t id = kInvalid;
-        float d = INFINITY;
-        int src = -1;
-        if (j < kq) {
-            if (job.order == 1) {
-                if (j < cnt) src = static_cast<int>(j);
-            } else if (j >= kq - cnt) {
-                src = static_cast<int>(size - kq + j);
-            }
+        const bool fill = j >= kq || (job.order == 1 ? j >= cnt : j < kq - cnt);
+        if (fill) {
+            oi[j] = kInvalid;
+            if (od) od[j] = INFINITY;
         }
-        if (src >= 0) {
-            const uint64_t kk = w.pool[src];
-            id = key_id(kk);
-            d = key_dist(kk);
+    }
+    const uint32_t first = job.order == 1 ? 0u : size - cnt;  // pool entries [first, first+cnt)
+    const uint32_t dst0 = job.order == 1 ? 0u : kq - cnt;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const uint32_t i = static_cast<uint32_t>(s * kWave) + lane;
+        if (i >= first && i < first + cnt) {
+            oi[dst0 + i - first] = key_id(P[s]);
+            if (od) od[dst0 + i - first] = key_dist(P[s]);
         }
-        oi[j] = id;
-        if (od) od[j] = d;
     }
     if (job.out_stats && lane == 0) {
         job.out_stats[2 * static_cast<size_t>(q)] = n_exp;
@@ -326,10 +354,12 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
     wave_sync();
 }
 
+// Body shared by the two kernel variants: SQ8 params to LDS, then a
+// persistent loop pulling query indices from the work counter.
 template <int EF_CAP>
-__global__ void __launch_bounds__(256) h1_search_kernel(DevIndex ix, SearchJob job,
-                                                         uint32_t cache_bits, uint32_t dim_pad,
-                                                         uint32_t wave_bytes) {
+__device__ __forceinline__ void search_kernel_body(const DevIndex& ix, const SearchJob& job,
+                                                   uint32_t cache_bits, uint32_t dim_pad,
+                                                   uint32_t wave_bytes) {
     extern __shared__ __align__(16) unsigned char lds[];
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wid = threadIdx.x / kWave;
@@ -344,10 +374,8 @@ __global__ void __launch_bounds__(256) h1_search_kernel(DevIndex ix, SearchJob j
     unsigned char* wb = lds + 8u * dim_pad + static_cast<size_t>(wid) * wave_bytes;
     WaveLds w;
     w.qv = reinterpret_cast<float*>(wb);
-    w.pool = reinterpret_cast<uint64_t*>(wb + 4u * dim_pad);
-    w.bkeys = w.pool + EF_CAP;
-    w.bsorted = w.bkeys + kWave;
-    w.bids = reinterpret_cast<uint32_t*>(w.bsorted + kWave);
+    w.stage = reinterpret_cast<uint64_t*>(wb + 4u * dim_pad);
+    w.bids = reinterpret_cast<uint32_t*>(w.stage + EF_CAP);
     w.cache = w.bids + kWave;
 
     for (;;) {
@@ -359,40 +387,47 @@ __global__ void __launch_bounds__(256) h1_search_kernel(DevIndex ix, SearchJob j
     }
 }
 
-__global__ void reset_counter_kernel(uint32_t* counter) { *counter = 0u; }
-
+// Register allocation left to the compiler.
 template <int EF_CAP>
-hipError_t launch_t(const DevIndex& ix, const SearchJob& job, const LaunchCfg& cfg,
-                    hipStream_t stream, size_t lds, uint32_t dim_pad, uint32_t wave_bytes,
-                    int groups) {
-    static size_t attr_bytes = 64 * 1024;
-    if (lds > attr_bytes) {
-        hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void*>(&h1_search_kernel<EF_CAP>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        if (e != hipSuccess) return e;
-        attr_bytes = lds;
-    }
-    hipLaunchKernelGGL(h1_search_kernel<EF_CAP>, dim3(groups), dim3(kWave * cfg.waves_per_group),
-                       lds, stream, ix, job, static_cast<uint32_t>(cfg.cache_bits), dim_pad,
-                       wave_bytes);
-    return hipGetLastError();
+__global__ void __launch_bounds__(256) h1_search_kernel(DevIndex ix, SearchJob job,
+                                                         uint32_t cache_bits, uint32_t dim_pad,
+                                                         uint32_t wave_bytes) {
+    search_kernel_body<EF_CAP>(ix, job, cache_bits, dim_pad, wave_bytes);
 }
 
+// Register file capped at 64 VGPRs so 8 waves fit per SIMD (32 per CU): more
+// queries in flight to hide memory latency, at the price of a few spills.
 template <int EF_CAP>
-int occupancy_t(int threads, size_t lds) {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &nb, reinterpret_cast<const void*>(&h1_search_kernel<EF_CAP>), threads, lds) !=
-        hipSuccess)
-        return 1;
-    return nb > 0 ? nb : 1;
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+h1_search_kernel_w8(DevIndex ix, SearchJob job, uint32_t cache_bits, uint32_t dim_pad,
+                    uint32_t wave_bytes) {
+    search_kernel_body<EF_CAP>(ix, job, cache_bits, dim_pad, wave_bytes);
+}
+
+__global__ void reset_counter_kernel(uint32_t* counter) { *counter = 0u; }
+
+using KernelFn = void (*)(DevIndex, SearchJob, uint32_t, uint32_t, uint32_t);
+
+template <int EF_CAP>
+KernelFn kernel_for(int occ) {
+    return occ == 8 ? &h1_search_kernel_w8<EF_CAP> : &h1_search_kernel<EF_CAP>;
+}
+
+KernelFn select_kernel(int ef_cap, int occ) {
+    switch (ef_cap) {
+        case 64: return kernel_for<64>(occ);
+        case 128: return kernel_for<128>(occ);
+        case 256: return kernel_for<256>(occ);
+        case 512: return kernel_for<512>(occ);
+        case 1024: return kernel_for<1024>(occ);
+        default: return nullptr;
+    }
 }
 
 void geometry(const DevIndex& ix, const LaunchCfg& cfg, uint32_t& dim_pad, uint32_t& wave_bytes) {
     dim_pad = (ix.dim + 3u) & ~3u;
-    size_t wb = 4u * dim_pad + 8u * static_cast<size_t>(cfg.ef_cap) + 8u * kWave * 2 +
-                4u * kWave + (4u << cfg.cache_bits);
+    size_t wb = 4u * dim_pad + 8u * static_cast<size_t>(cfg.ef_cap) + 4u * kWave +
+                (4u << cfg.cache_bits);
     wave_bytes = static_cast<uint32_t>((wb + 15u) & ~static_cast<size_t>(15u));
 }
 
@@ -410,35 +445,38 @@ int search_full_grid(const DevIndex& ix, const LaunchCfg& cfg) {
     if (hipGetDevice(&dev) != hipSuccess) return 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 1;
-    const size_t lds = search_lds_bytes(ix, cfg);
-    const int threads = kWave * cfg.waves_per_group;
-    int per_cu = 1;
-    switch (cfg.ef_cap) {
-        case 64: per_cu = occupancy_t<64>(threads, lds); break;
-        case 128: per_cu = occupancy_t<128>(threads, lds); break;
-        case 256: per_cu = occupancy_t<256>(threads, lds); break;
-        case 512: per_cu = occupancy_t<512>(threads, lds); break;
-        default: per_cu = occupancy_t<1024>(threads, lds); break;
-    }
+    KernelFn k = select_kernel(cfg.ef_cap, cfg.occupancy);
+    if (!k) return 1;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k),
+                                                     kWave * cfg.waves_per_group,
+                                                     search_lds_bytes(ix, cfg)) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
     return per_cu * cus;
 }
 
 hipError_t launch_search(const DevIndex& ix, const SearchJob& job, const LaunchCfg& cfg,
                          hipStream_t stream) {
+    KernelFn k = select_kernel(cfg.ef_cap, cfg.occupancy);
+    if (!k || cfg.cache_bits < 4 || cfg.cache_bits > 14 || cfg.waves_per_group < 1 ||
+        cfg.waves_per_group > 4)
+        return hipErrorInvalidValue;
     uint32_t dim_pad, wave_bytes;
     geometry(ix, cfg, dim_pad, wave_bytes);
     const size_t lds = search_lds_bytes(ix, cfg);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
     int groups = cfg.groups > 0 ? cfg.groups : search_full_grid(ix, cfg);
     const int need = static_cast<int>((job.nq + cfg.waves_per_group - 1) / cfg.waves_per_group);
     if (groups > need) groups = need > 0 ? need : 1;
-    switch (cfg.ef_cap) {
-        case 64: return launch_t<64>(ix, job, cfg, stream, lds, dim_pad, wave_bytes, groups);
-        case 128: return launch_t<128>(ix, job, cfg, stream, lds, dim_pad, wave_bytes, groups);
-        case 256: return launch_t<256>(ix, job, cfg, stream, lds, dim_pad, wave_bytes, groups);
-        case 512: return launch_t<512>(ix, job, cfg, stream, lds, dim_pad, wave_bytes, groups);
-        case 1024: return launch_t<1024>(ix, job, cfg, stream, lds, dim_pad, wave_bytes, groups);
-        default: return hipErrorInvalidValue;
-    }
+    hipLaunchKernelGGL(k, dim3(groups), dim3(kWave * cfg.waves_per_group), lds, stream, ix, job,
+                       static_cast<uint32_t>(cfg.cache_bits), dim_pad, wave_bytes);
+    return hipGetLastError();
 }
 
 hipError_t launch_reset_counter(uint32_t* counter, hipStream_t stream) {

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "lqg_index.h"
@@ -66,7 +67,35 @@ lqg_status ensure_device(lqg_index* h) {
         sc[i] = h->sq8[2 * i + 1];
     }
     LQG_HIP_TRY(hipMemcpy(h->d_qdata.p, h->qdata.data(), nd, hipMemcpyHostToDevice));
-    LQG_HIP_TRY(hipMemcpy(h->d_nbr.p, h->nbr.data(), ng * 4, hipMemcpyHostToDevice));
+    // Device copy of the neighbour table: a repeated id later in a row, or the
+    // row's own node, is replaced by kInvalid.  The search skips both exactly as
+    // the reference does (the node is already visited when its row is scanned),
+    // so the kernel needs no per-expansion duplicate test.
+    std::vector<uint32_t> dev_nbr(h->nbr);
+    {
+        const uint64_t g = h->deg;
+        const unsigned nt = std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (unsigned t = 0; t < nt; ++t)
+            pool.emplace_back([&, t] {
+                for (uint64_t i = t; i < h->n; i += nt) {
+                    uint32_t* row = dev_nbr.data() + i * g;
+                    for (uint64_t a = 0; a < g; ++a) {
+                        if (row[a] == i) {
+                            row[a] = kInvalid;
+                            continue;
+                        }
+                        for (uint64_t b = 0; b < a; ++b)
+                            if (row[b] == row[a]) {
+                                row[a] = kInvalid;
+                                break;
+                            }
+                    }
+                }
+            });
+        for (auto& th : pool) th.join();
+    }
+    LQG_HIP_TRY(hipMemcpy(h->d_nbr.p, dev_nbr.data(), ng * 4, hipMemcpyHostToDevice));
     LQG_HIP_TRY(hipMemcpy(h->d_min.p, mn.data(), h->dim * 4, hipMemcpyHostToDevice));
     LQG_HIP_TRY(hipMemcpy(h->d_scale.p, sc.data(), h->dim * 4, hipMemcpyHostToDevice));
     LQG_HIP_TRY(h->d_counter.reserve(1));
@@ -74,18 +103,25 @@ lqg_status ensure_device(lqg_index* h) {
     return LQG_OK;
 }
 
-namespace {
+int class_slot(int ef_cap) {
+    switch (ef_cap) {
+        case 64: return 0;
+        case 128: return 1;
+        case 256: return 2;
+        case 512: return 3;
+        default: return 4;
+    }
+}
 
-LaunchCfg default_cfg(uint32_t ef_max, uint64_t nq) {
+LaunchCfg default_launch(int ef_cap) {
     LaunchCfg c;
-    c.ef_cap = ef_class(ef_max);
-    c.waves_per_group = nq >= 4 ? 4 : 1;
+    c.ef_cap = ef_cap;
+    c.waves_per_group = 4;
+    c.occupancy = 0;
     c.cache_bits = 10;
     c.groups = 0;
     return c;
 }
-
-}  // namespace
 
 lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
                      const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, lqg_order order,
@@ -105,7 +141,10 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
     job.out_d = d_dists;
     job.out_stats = d_stats;
     job.counter = h->d_counter.p;
-    const LaunchCfg cfg = default_cfg(ef_max, nq);
+    const int cls = ef_class(ef_max);
+    LaunchCfg cfg = h->launch_set[class_slot(cls)] ? h->launch[class_slot(cls)] : default_launch(cls);
+    cfg.ef_cap = cls;
+    if (nq < static_cast<uint64_t>(cfg.waves_per_group)) cfg.waves_per_group = 1;
     LQG_HIP_TRY(launch_reset_counter(h->d_counter.p, s));
     LQG_HIP_TRY(launch_search(h->dev(), job, cfg, s));
     return LQG_OK;
@@ -382,6 +421,114 @@ lqg_status lqg_search_batch_device(lqg_index* h, const float* d_queries, uint64_
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : h->stream;
     return lqg::run_batch(h, d_queries, nq, d_ef, d_k, ef_max, kmax, order, d_results, d_dists, d_stats,
                      s);
+}
+
+lqg_status lqg_set_launch(lqg_index* h, uint32_t ef, const lqg_launch* cfg) {
+    if (!h) return fail(LQG_ERR_ARG, "null index");
+    if (ef != 0 && ef_class(ef) == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024");
+    if (cfg && (cfg->waves_per_group < 1 || cfg->waves_per_group > 4 || cfg->cache_bits < 4 ||
+                cfg->cache_bits > 14 || (cfg->occupancy != 0 && cfg->occupancy != 8) ||
+                cfg->groups < 0))
+        return fail(LQG_ERR_ARG, "launch settings out of range");
+    for (int slot = 0; slot < 5; ++slot) {
+        const int cap = 64 << slot;
+        if (ef != 0 && ef_class(ef) != cap) continue;
+        if (!cfg) {
+            h->launch_set[slot] = false;
+            continue;
+        }
+        lqg::LaunchCfg c;
+        c.ef_cap = cap;
+        c.waves_per_group = cfg->waves_per_group;
+        c.cache_bits = cfg->cache_bits;
+        c.occupancy = cfg->occupancy;
+        c.groups = cfg->groups;
+        h->launch[slot] = c;
+        h->launch_set[slot] = true;
+    }
+    return LQG_OK;
+}
+
+lqg_status lqg_get_launch(const lqg_index* h, uint32_t ef, lqg_launch* cfg) {
+    if (!h || !cfg) return fail(LQG_ERR_ARG, "null argument");
+    const int cls = ef_class(ef == 0 ? 1 : ef);
+    if (cls == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024");
+    const int slot = lqg::class_slot(cls);
+    const lqg::LaunchCfg c = h->launch_set[slot] ? h->launch[slot] : lqg::default_launch(cls);
+    cfg->waves_per_group = c.waves_per_group;
+    cfg->cache_bits = c.cache_bits;
+    cfg->occupancy = c.occupancy;
+    cfg->groups = c.groups;
+    return LQG_OK;
+}
+
+lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_t ef, uint32_t k,
+                        lqg_launch* best) {
+    if (!h || !queries) return fail(LQG_ERR_ARG, "null argument");
+    if (nq == 0 || nq > 0xFFFFFFFFull || k == 0 || ef == 0) return fail(LQG_ERR_ARG, "empty tuning set");
+    const int cls = ef_class(ef);
+    if (cls == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024");
+    lqg_status st = check_queries(queries, nq * h->dim);
+    if (st != LQG_OK) return st;
+    st = lqg::ensure_device(h);
+    if (st != LQG_OK) return st;
+    lqg::DevBuf<float> dq;
+    lqg::DevBuf<uint32_t> dids;
+    HIP_TRY(dq.reserve(nq * h->dim));
+    HIP_TRY(dids.reserve(nq * k));
+    HIP_TRY(hipMemcpy(dq.p, queries, nq * h->dim * 4, hipMemcpyHostToDevice));
+    const int slot = lqg::class_slot(cls);
+    const bool had = h->launch_set[slot];
+    const lqg::LaunchCfg saved = h->launch[slot];
+    float best_ms = 1e30f;
+    lqg::LaunchCfg best_cfg = lqg::default_launch(cls);
+    const int occs[] = {0, 8};
+    const int bits[] = {8, 9, 10, 11};
+    const int wpg[] = {1, 2, 4};
+    for (int occ : occs)
+        for (int cb : bits)
+            for (int wg : wpg) {
+                lqg::LaunchCfg c = lqg::default_launch(cls);
+                c.occupancy = occ;
+                c.cache_bits = cb;
+                c.waves_per_group = wg;
+                h->launch[slot] = c;
+                h->launch_set[slot] = true;
+                float ms_sum = 0.f;
+                bool ok = true;
+                for (int rep = 0; rep < 3 && ok; ++rep) {
+                    if (hipEventRecord(h->ev0, h->stream) != hipSuccess) ok = false;
+                    if (ok && lqg::run_batch(h, dq.p, nq, nullptr, nullptr, ef, k, LQG_ORDER_NEAREST,
+                                             dids.p, nullptr, nullptr, h->stream) != LQG_OK)
+                        ok = false;
+                    if (ok && hipEventRecord(h->ev1, h->stream) != hipSuccess) ok = false;
+                    if (ok && hipEventSynchronize(h->ev1) != hipSuccess) ok = false;
+                    float ms = 0.f;
+                    if (ok && hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess) ok = false;
+                    if (rep > 0) ms_sum += ms;  // the first run warms up
+                }
+                (void)hipGetLastError();
+                if (ok && ms_sum / 2 < best_ms) {
+                    best_ms = ms_sum / 2;
+                    best_cfg = c;
+                }
+            }
+    dq.release();
+    dids.release();
+    if (best_ms >= 1e30f) {
+        h->launch[slot] = saved;
+        h->launch_set[slot] = had;
+        return fail(LQG_ERR_DEVICE, "no launch setting ran");
+    }
+    h->launch[slot] = best_cfg;
+    h->launch_set[slot] = true;
+    if (best) {
+        best->waves_per_group = best_cfg.waves_per_group;
+        best->cache_bits = best_cfg.cache_bits;
+        best->occupancy = best_cfg.occupancy;
+        best->groups = best_cfg.groups;
+    }
+    return LQG_OK;
 }
 
 lqg_status lqg_info(const lqg_index* h, uint64_t* num_elements, uint64_t* dimension,

@@ -46,6 +46,12 @@ struct DevBuf {
 // Smallest pool-capacity class that holds ef, 0 if none does.
 int ef_class(uint32_t ef);
 
+// Index 0..4 of a capacity class (64..1024).
+int class_slot(int ef_cap);
+
+// Launch settings used when none were set for a class.
+LaunchCfg default_launch(int ef_cap);
+
 }  // namespace lqg
 
 struct lqg_index {
@@ -73,6 +79,9 @@ struct lqg_index {
     lqg::DevBuf<uint32_t> d_ids, d_ef, d_k, d_stats, d_counter;
     lqg::DevBuf<float> d_dist;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // launch settings per pool-capacity class (64, 128, 256, 512, 1024)
+    lqg::LaunchCfg launch[5];
+    bool launch_set[5] = {false, false, false, false, false};
 
     lqg::DevIndex dev() const {
         lqg::DevIndex ix;

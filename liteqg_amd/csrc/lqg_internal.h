@@ -36,8 +36,9 @@ struct SearchJob {
 struct LaunchCfg {
     int ef_cap;          // pool capacity class: 64, 128, 256, 512 or 1024
     int waves_per_group; // waves per workgroup
-    int cache_bits;      // log2 of the per-wave visited cache
+    int cache_bits;      // log2 of the per-wave visited cache (4..14)
     int groups;          // workgroups in the persistent grid (0 = fill the device)
+    int occupancy;       // 8: kernel variant capped at 64 VGPRs (8 waves/SIMD); 0: uncapped
 };
 
 // Returns LDS bytes one workgroup needs for this configuration.

@@ -23,7 +23,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import ORDER, LqgError, Stats, check, lib, ptr
+from ._lib import ORDER, Launch, LqgError, Stats, check, lib, ptr
 
 UINT_MAX = 0xFFFFFFFF
 
@@ -128,6 +128,27 @@ class Index:
             out.append(dict(queries=st.queries, expansions=st.expansions,
                             evaluations=st.evaluations, device_ms=st.device_ms))
         return out[0] if len(out) == 1 else tuple(out)
+
+    def set_launch(self, ef: int = 0, waves_per_group: int = 4, cache_bits: int = 10,
+                   occupancy: int = 0, groups: int = 0) -> None:
+        """Kernel launch settings for ef's capacity class (ef=0: every class)."""
+        cfg = Launch(waves_per_group, cache_bits, occupancy, groups)
+        check(lib().lqg_set_launch(self._h, int(ef), C.byref(cfg)))
+
+    def reset_launch(self, ef: int = 0) -> None:
+        check(lib().lqg_set_launch(self._h, int(ef), None))
+
+    def get_launch(self, ef: int) -> dict:
+        cfg = Launch()
+        check(lib().lqg_get_launch(self._h, int(ef), C.byref(cfg)))
+        return {f: getattr(cfg, f) for f, _ in Launch._fields_}
+
+    def autotune(self, queries, ef: int, k: int = 10) -> dict:
+        """Time the candidate launch settings on these queries; keep the fastest."""
+        Q = np.ascontiguousarray(queries, dtype=np.float32).reshape(-1, self.dimension)
+        best = Launch()
+        check(lib().lqg_autotune(self._h, ptr(Q), Q.shape[0], int(ef), int(k), C.byref(best)))
+        return {f: getattr(best, f) for f, _ in Launch._fields_}
 
     def info(self) -> dict:
         n, d, g = C.c_uint64(), C.c_uint64(), C.c_uint64()

@@ -155,3 +155,32 @@ def test_recall_small_knn_graph(oracle_mod, tmp_path):
     gt, _ = oracle_mod.bruteforce_fp64(base, Q, 10)
     ids = ix.search_batch(Q, 10, ef=128, order="nearest")
     assert oracle_mod.recall_at_k(ids, gt, 10) > 0.9
+
+
+@pytest.mark.parametrize("occ,bits,wpg", [(0, 4, 1), (0, 10, 4), (8, 9, 4), (8, 4, 2), (0, 14, 1)])
+def test_launch_settings_do_not_change_results(oracle_mod, tmp_path, occ, bits, wpg):
+    """Occupancy variant, cache size and group shape change speed only.  A tiny
+    cache (16 entries) forgets almost everything, so most revisits are caught by
+    the pool test instead of the cache."""
+    arr, base = make_random_index(6000, 64, 32, seed=11, knn=True)
+    ix = device_from_arrays(arr, tmp_path)
+    rng = np.random.default_rng(4)
+    Q = (base[rng.integers(0, 6000, 200)] + rng.normal(0, 0.05, size=(200, 64))).astype(np.float32)
+    ix.set_launch(0, waves_per_group=wpg, cache_bits=bits, occupancy=occ)
+    assert ix.get_launch(100)["cache_bits"] == bits
+    for ef in (10, 100, 300):
+        ids, dis, st = ix.search_batch(Q, 10, ef=ef, order="nearest", return_distances=True,
+                                       return_stats=True)
+        exp, exp_d, est = oracle_batch(oracle_mod, arr, Q, ef, 10, "nearest")
+        assert_same(ids, dis, exp, exp_d)
+        assert st["expansions"] == int(est[:, 0].sum())
+
+
+def test_autotune_keeps_results(oracle_mod, tmp_path):
+    arr, base = make_random_index(8000, 128, 32, seed=12, kind="int", knn=True)
+    ix = device_from_arrays(arr, tmp_path)
+    Q = base[:400] + 3.0
+    before = ix.search_batch(Q, 10, ef=80, order="nearest")
+    best = ix.autotune(Q, 80, 10)
+    assert ix.get_launch(80) == best
+    assert np.array_equal(ix.search_batch(Q, 10, ef=80, order="nearest"), before)
