@@ -188,6 +188,45 @@ __device__ __forceinline__ uint64_t eval_fresh(const DevIndex& ix, const float* 
     return leader ? make_key(r, bids[c]) : ~0ull;
 }
 
+// Exact integer evaluation.  When every SQ8 parameter is min 0 / scale 1 the
+// decode is the identity, and when the query is integer valued in [0, 255]
+// every term of the reference's FP32 sum is an integer: t = q - c is exact,
+// t*t <= 65025, and with dim <= 256 every partial and pairwise sum stays below
+// 2^24, so each FP32 operation is exact and the reference's result is exactly
+// sum (q_i - c_i)^2 whatever the order.  That integer is computed here as
+// sum q^2 + sum c^2 - 2 sum q*c with v_dot4_u32_u8 and converted to float
+// (exact below 2^24): bit-identical, with one lane per candidate.
+__device__ __forceinline__ uint64_t eval_fresh_int(const DevIndex& ix, const uint32_t* qp,
+                                                   uint32_t qq, const uint32_t* bids, uint32_t m,
+                                                   uint32_t lane) {
+    if (lane >= m) return ~0ull;
+    const uint32_t id = bids[lane];
+    const uint4* row = reinterpret_cast<const uint4*>(ix.qdata + static_cast<size_t>(id) * ix.dim);
+    const uint32_t n16 = ix.dim >> 4;
+    uint32_t qc = 0, cc = 0;
+    for (uint32_t base = 0; base < n16; base += 8) {
+        uint4 w[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (base + c < n16) w[c] = row[base + c];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            if (base + c < n16) {
+                const uint4 qw = reinterpret_cast<const uint4*>(qp)[base + c];
+                qc = __builtin_amdgcn_udot4(qw.x, w[c].x, qc, false);
+                qc = __builtin_amdgcn_udot4(qw.y, w[c].y, qc, false);
+                qc = __builtin_amdgcn_udot4(qw.z, w[c].z, qc, false);
+                qc = __builtin_amdgcn_udot4(qw.w, w[c].w, qc, false);
+                cc = __builtin_amdgcn_udot4(w[c].x, w[c].x, cc, false);
+                cc = __builtin_amdgcn_udot4(w[c].y, w[c].y, cc, false);
+                cc = __builtin_amdgcn_udot4(w[c].z, w[c].z, cc, false);
+                cc = __builtin_amdgcn_udot4(w[c].w, w[c].w, cc, false);
+            }
+        }
+    }
+    return make_key(static_cast<float>(qq + cc - 2u * qc), id);
+}
+
 struct WaveLds {
     float* qv;        // query (dim)
     uint64_t* stage;  // pool permutation staging (EF_CAP)
@@ -205,7 +244,33 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
         max(1u, min(job.ef ? job.ef[q] : job.ef_all, static_cast<uint32_t>(EF_CAP)));
     const uint32_t kq = min(job.k ? job.k[q] : job.k_all, job.kstride);
     const float* qsrc = job.queries + static_cast<size_t>(q) * ix.dim;
-    for (uint32_t i = lane; i < ix.dim; i += kWave) w.qv[i] = qsrc[i];
+    bool qint = true;
+    for (uint32_t i = lane; i < ix.dim; i += kWave) {
+        const float v = qsrc[i];
+        w.qv[i] = v;
+        qint = qint && v >= 0.f && v <= 255.f && v == rintf(v);
+    }
+    // exact integer path (see eval_fresh_int): identity SQ8, integer query
+    const bool int_path = ix.sq8_identity && (ix.dim & 15u) == 0 && ix.dim <= 256 &&
+                          __ballot(!qint) == 0;
+    uint32_t qq = 0;
+    const uint32_t* qp = reinterpret_cast<const uint32_t*>(w.qv);
+    if (int_path) {
+        wave_sync();
+        // pack the query to bytes in place: word j <- dims 4j..4j+3
+        uint32_t word = 0;
+        if (lane < ix.dim / 4) {
+            word = static_cast<uint32_t>(w.qv[4 * lane]) |
+                   (static_cast<uint32_t>(w.qv[4 * lane + 1]) << 8) |
+                   (static_cast<uint32_t>(w.qv[4 * lane + 2]) << 16) |
+                   (static_cast<uint32_t>(w.qv[4 * lane + 3]) << 24);
+            qq = __builtin_amdgcn_udot4(word, word, 0u, false);
+        }
+        wave_sync();
+        if (lane < ix.dim / 4) w.qv[lane] = __uint_as_float(word);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) qq += __shfl_xor(qq, o);
+    }
     const uint32_t csize = 1u << cache_bits;
     for (uint32_t i = lane; i < csize; i += kWave) w.cache[i] = kInvalid;
     if (lane == 0) {
@@ -219,7 +284,10 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
 #pragma unroll
     for (int s = 0; s < S; ++s) P[s] = ~0ull;
     {
-        const uint64_t k0 = readlane64(eval_fresh(ix, w.qv, pmin, pscale, w.bids, 1, lane), 0);
+        const uint64_t k0 =
+            readlane64(int_path ? eval_fresh_int(ix, qp, qq, w.bids, 1, lane)
+                                : eval_fresh(ix, w.qv, pmin, pscale, w.bids, 1, lane),
+                       0);
         if (lane == 0) P[0] = k0;
     }
     uint32_t size = 1, n_exp = 0, n_eval = 1;
@@ -262,7 +330,8 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
             if (fresh) w.bids[lanes_below(fm)] = nid;
             wave_sync();
             n_eval += m;
-            const uint64_t key = eval_fresh(ix, w.qv, pmin, pscale, w.bids, m, lane);
+            const uint64_t key = int_path ? eval_fresh_int(ix, qp, qq, w.bids, m, lane)
+                                          : eval_fresh(ix, w.qv, pmin, pscale, w.bids, m, lane);
 
             // admission (qg.hpp:329-335): pool not full, or below its maximum
             uint64_t last = 0;

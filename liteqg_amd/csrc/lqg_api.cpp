@@ -62,9 +62,11 @@ lqg_status ensure_device(lqg_index* h) {
     LQG_HIP_TRY(h->d_min.reserve(h->dim));
     LQG_HIP_TRY(h->d_scale.reserve(h->dim));
     std::vector<float> mn(h->dim), sc(h->dim);
+    h->sq8_identity = true;
     for (uint64_t i = 0; i < h->dim; ++i) {
         mn[i] = h->sq8[2 * i];
         sc[i] = h->sq8[2 * i + 1];
+        h->sq8_identity = h->sq8_identity && mn[i] == 0.0f && sc[i] == 1.0f;
     }
     LQG_HIP_TRY(hipMemcpy(h->d_qdata.p, h->qdata.data(), nd, hipMemcpyHostToDevice));
     // Device copy of the neighbour table: a repeated id later in a row, or the

@@ -70,6 +70,7 @@ struct lqg_index {
 
     // device copy
     bool on_device = false;
+    bool sq8_identity = false;  // all SQ8 params are min 0, scale 1
     hipStream_t stream = nullptr;
     lqg::DevBuf<uint8_t> d_qdata;
     lqg::DevBuf<uint32_t> d_nbr;
@@ -93,6 +94,7 @@ struct lqg_index {
         ix.dim = static_cast<uint32_t>(dim);
         ix.deg = static_cast<uint32_t>(deg);
         ix.ep = ep;
+        ix.sq8_identity = sq8_identity ? 1 : 0;
         return ix;
     }
 };

@@ -17,6 +17,7 @@ struct DevIndex {
     const float* sq8_min;   // dim
     const float* sq8_scale; // dim
     uint32_t n, dim, deg, ep;
+    int sq8_identity;       // every dim has min 0 and scale 1 (decode is exact)
 };
 
 // One batch of queries for the search kernel.

@@ -184,3 +184,28 @@ def test_autotune_keeps_results(oracle_mod, tmp_path):
     best = ix.autotune(Q, 80, 10)
     assert ix.get_launch(80) == best
     assert np.array_equal(ix.search_batch(Q, 10, ef=80, order="nearest"), before)
+
+
+@pytest.mark.parametrize("d", [128, 64, 256, 272])
+def test_integer_path_matches_oracle(oracle_mod, tmp_path, d):
+    """Integer data with min 0 / scale 1 SQ8 params and integer queries take the
+    v_dot4 path (d <= 256); the result must be bit-identical to the FP32 order
+    of the reference.  Queries with a fractional or out-of-range value in the
+    same batch take the FP32 path."""
+    arr, base = make_random_index(12000, d, 32, seed=d, kind="int", knn=True)
+    assert (arr.params[:, 0] == 0).all() and (arr.params[:, 1] == 1).all()
+    ix = device_from_arrays(arr, tmp_path)
+    rng = np.random.default_rng(d)
+    Q = np.clip(np.rint(base[rng.integers(0, 12000, 96)] + rng.normal(0, 30, size=(96, d))),
+                0, 255).astype(np.float32)
+    Q[5, 3] += 0.5          # fractional: FP32 path
+    Q[9, 0] = 256.0         # out of byte range: FP32 path
+    Q[11, :] = 0.0          # all zeros: integer path
+    Q[12, :] = 255.0
+    for ef in (16, 64, 200):
+        for order in ("nearest", "reference"):
+            ids, dis, st = ix.search_batch(Q, 10, ef=ef, order=order, return_distances=True,
+                                           return_stats=True)
+            exp, exp_d, est = oracle_batch(oracle_mod, arr, Q, ef, 10, order)
+            assert_same(ids, dis, exp, exp_d)
+            assert st["expansions"] == int(est[:, 0].sum())
