@@ -176,6 +176,9 @@ def main():
     ap.add_argument("--ef", type=int, default=0, help="skip the sweep and use this ef")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--plain", type=int, default=500, help="queries timed on the plain path")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="for rocprofv3 --pmc runs: build, 3 launches at --ef (default 50) with "
+                         "the tuned launch settings, then a 1 GiB device copy (counter scale)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the query path is data-parallel over queries and "
@@ -197,6 +200,19 @@ def main():
     ix.build_index(base, ef_build, num_iter=3)
     build_s = time.perf_counter() - t
     log(f"build {build_s:.1f}s")
+    if args.pmc_probe:
+        ef = args.ef or 50
+        db = DeviceBatch(ix, Q, k)
+        ix.autotune(Q, ef, k)
+        for _ in range(3):
+            db.launch(ef)
+        torch.cuda.synchronize()
+        x = torch.zeros(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)  # reads 1 GiB, writes 1 GiB
+        torch.cuda.synchronize()
+        log(f"pmc probe: 3 launches at ef={ef}, launch {ix.get_launch(ef)}, then a 1 GiB copy")
+        return
     t = time.perf_counter()
     gt = ground_truth(base, Q, k)
     log(f"ground truth {time.perf_counter() - t:.1f}s")
@@ -241,7 +257,15 @@ def main():
 
     # roofline: algorithmic bytes per query, SQ8 path
     #   4D (query) + 8D (SQ8 params) + 4*deg*E (id rows) + D*V (codes) + 4k (ids out)
-    alg_bytes = float(np.sum(4 * d + 8 * d + 4 * deg * E + d * V + 4 * k))
+    # with E and V the reference's counts: expansions and distinct nodes
+    # evaluated, from the exact counting launch (the timed kernel's own
+    # evaluation count V_eval also includes re-evaluations its lossy visited
+    # cache lets through).
+    exact = ix.count_visits(Q, chosen, k).astype(np.int64)
+    if not np.array_equal(exact[:, 0], E):
+        raise RuntimeError("expansion counts differ between the timed and the counting launch")
+    V_ref = exact[:, 1]
+    alg_bytes = float(np.sum(4 * d + 8 * d + 4 * deg * E + d * V_ref + 4 * k))
     achieved = alg_bytes / (ms_step / 1e3) / 1e9
     traffic = read_traffic()
 
@@ -282,7 +306,8 @@ def main():
                      "traffic": traffic,
                      "bytes_per_query": round(alg_bytes / args.nq, 1),
                      "expansions_per_query": round(float(E.mean()), 2),
-                     "evaluations_per_query": round(float(V.mean()), 2)},
+                     "distinct_evaluations_per_query": round(float(V_ref.mean()), 2),
+                     "kernel_evaluations_per_query": round(float(V.mean()), 2)},
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -127,6 +127,14 @@ lqg_status lqg_get_launch(const lqg_index* index, uint32_t ef, lqg_launch* cfg);
 lqg_status lqg_autotune(lqg_index* index, const float* queries, uint64_t nq, uint32_t ef,
                         uint32_t k, lqg_launch* best);
 
+/* Diagnostic: runs the batch search of nq queries (host memory) at (ef, k)
+ * with an exact N-bit visited bitmap per wavefront beside the kernel's lossy
+ * cache, and writes per query (E, V) to stats (2 x nq uint32): expansions, and
+ * distinct nodes evaluated -- the counts of the reference's visited set
+ * (qg.hpp:320-328).  Used for algorithmic-byte accounting; not for speed. */
+lqg_status lqg_count_visits(lqg_index* index, const float* queries, uint64_t nq, uint32_t ef,
+                            uint32_t k, uint32_t* stats);
+
 /* Read-only views of the loaded index (host copies). */
 lqg_status lqg_info(const lqg_index* index, uint64_t* num_elements, uint64_t* dimension,
                     uint64_t* degree_bound, uint32_t* entry_point, uint32_t* ef);

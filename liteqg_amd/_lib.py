@@ -52,6 +52,7 @@ _SIGS = {
     "lqg_set_launch": [_P, C.c_uint32, _P],
     "lqg_get_launch": [_P, C.c_uint32, _P],
     "lqg_autotune": [_P, _P, C.c_uint64, C.c_uint32, C.c_uint32, _P],
+    "lqg_count_visits": [_P, _P, C.c_uint64, C.c_uint32, C.c_uint32, _P],
     "lqg_info": [_P, _P, _P, _P, _P, _P],
     "lqg_get_graph": [_P, _P],
 }

@@ -234,10 +234,15 @@ struct WaveLds {
     uint32_t* cache;  // visited cache (1 << cache_bits)
 };
 
+// Marks node x in the counting bitmap; true when it was not marked before.
+__device__ __forceinline__ bool first_visit(uint32_t* bits, uint32_t x) {
+    return (atomicOr(&bits[x >> 5], 1u << (x & 31u)) & (1u << (x & 31u))) == 0;
+}
+
 template <int EF_CAP>
 __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
                            const float* pmin, const float* pscale, const WaveLds& w,
-                           uint32_t cache_bits, uint32_t lane) {
+                           uint32_t cache_bits, uint32_t lane, uint32_t* vbits) {
     constexpr int S = EF_CAP / kWave;
     // clamps keep register, LDS and output indexing in bounds whatever the arrays hold
     const uint32_t ef =
@@ -276,6 +281,11 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
     if (lane == 0) {
         w.bids[0] = ix.ep;
         w.cache[(ix.ep * 2654435761u) >> (32 - cache_bits)] = ix.ep;
+    }
+    if (vbits) {
+        for (uint32_t i = lane; i < job.visit_words; i += kWave) vbits[i] = 0u;
+        __threadfence_block();
+        if (lane == 0) first_visit(vbits, ix.ep);
     }
     wave_sync();
 
@@ -329,7 +339,12 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
             const uint32_t m = static_cast<uint32_t>(__popcll(fm));
             if (fresh) w.bids[lanes_below(fm)] = nid;
             wave_sync();
-            n_eval += m;
+            if (vbits) {
+                const bool first = fresh && first_visit(vbits, nid);
+                n_eval += static_cast<uint32_t>(__popcll(__ballot(first)));
+            } else {
+                n_eval += m;
+            }
             const uint64_t key = int_path ? eval_fresh_int(ix, qp, qq, w.bids, m, lane)
                                           : eval_fresh(ix, w.qv, pmin, pscale, w.bids, m, lane);
 
@@ -447,12 +462,16 @@ __device__ __forceinline__ void search_kernel_body(const DevIndex& ix, const Sea
     w.bids = reinterpret_cast<uint32_t*>(w.stage + EF_CAP);
     w.cache = w.bids + kWave;
 
+    uint32_t* vbits = job.visit_bits
+                          ? job.visit_bits + static_cast<size_t>(blockIdx.x * (blockDim.x / kWave) + wid) *
+                                                 job.visit_words
+                          : nullptr;
     for (;;) {
         uint32_t q = 0;
         if (lane == 0) q = atomicAdd(job.counter, 1u);
         q = __shfl(q, 0);
         if (q >= job.nq) break;
-        search_one<EF_CAP>(ix, job, q, pmin, pscale, w, cache_bits, lane);
+        search_one<EF_CAP>(ix, job, q, pmin, pscale, w, cache_bits, lane, vbits);
     }
 }
 
@@ -525,6 +544,13 @@ int search_full_grid(const DevIndex& ix, const LaunchCfg& cfg) {
     return per_cu * cus;
 }
 
+int search_groups(const DevIndex& ix, const SearchJob& job, const LaunchCfg& cfg) {
+    int groups = cfg.groups > 0 ? cfg.groups : search_full_grid(ix, cfg);
+    const int need = static_cast<int>((job.nq + cfg.waves_per_group - 1) / cfg.waves_per_group);
+    if (groups > need) groups = need > 0 ? need : 1;
+    return groups;
+}
+
 hipError_t launch_search(const DevIndex& ix, const SearchJob& job, const LaunchCfg& cfg,
                          hipStream_t stream) {
     KernelFn k = select_kernel(cfg.ef_cap, cfg.occupancy);
@@ -540,9 +566,7 @@ hipError_t launch_search(const DevIndex& ix, const SearchJob& job, const LaunchC
                                            static_cast<int>(lds));
         if (e != hipSuccess) return e;
     }
-    int groups = cfg.groups > 0 ? cfg.groups : search_full_grid(ix, cfg);
-    const int need = static_cast<int>((job.nq + cfg.waves_per_group - 1) / cfg.waves_per_group);
-    if (groups > need) groups = need > 0 ? need : 1;
+    const int groups = search_groups(ix, job, cfg);
     hipLaunchKernelGGL(k, dim3(groups), dim3(kWave * cfg.waves_per_group), lds, stream, ix, job,
                        static_cast<uint32_t>(cfg.cache_bits), dim_pad, wave_bytes);
     return hipGetLastError();

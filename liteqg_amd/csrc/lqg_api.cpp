@@ -127,7 +127,8 @@ LaunchCfg default_launch(int ef_cap) {
 
 lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
                      const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, lqg_order order,
-                     uint32_t* d_results, float* d_dists, uint32_t* d_stats, hipStream_t s) {
+                     uint32_t* d_results, float* d_dists, uint32_t* d_stats, hipStream_t s,
+                     bool count_visits) {
     if (ef_class(ef_max) == 0)
         return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported on the device");
     SearchJob job;
@@ -143,10 +144,20 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
     job.out_d = d_dists;
     job.out_stats = d_stats;
     job.counter = h->d_counter.p;
+    job.visit_bits = nullptr;
+    job.visit_words = 0;
     const int cls = ef_class(ef_max);
     LaunchCfg cfg = h->launch_set[class_slot(cls)] ? h->launch[class_slot(cls)] : default_launch(cls);
     cfg.ef_cap = cls;
     if (nq < static_cast<uint64_t>(cfg.waves_per_group)) cfg.waves_per_group = 1;
+    if (count_visits) {
+        // one N-bit bitmap per wave of the grid
+        const int groups = search_groups(h->dev(), job, cfg);
+        job.visit_words = static_cast<uint32_t>((h->n + 31) / 32);
+        const size_t words = static_cast<size_t>(groups) * cfg.waves_per_group * job.visit_words;
+        LQG_HIP_TRY(h->d_visit.reserve(words));
+        job.visit_bits = h->d_visit.p;
+    }
     LQG_HIP_TRY(launch_reset_counter(h->d_counter.p, s));
     LQG_HIP_TRY(launch_search(h->dev(), job, cfg, s));
     return LQG_OK;
@@ -212,6 +223,7 @@ lqg_status lqg_destroy(lqg_index* h) {
     h->d_stats.release();
     h->d_counter.release();
     h->d_dist.release();
+    h->d_visit.release();
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -530,6 +542,30 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
         best->occupancy = best_cfg.occupancy;
         best->groups = best_cfg.groups;
     }
+    return LQG_OK;
+}
+
+lqg_status lqg_count_visits(lqg_index* h, const float* queries, uint64_t nq, uint32_t ef,
+                            uint32_t k, uint32_t* stats) {
+    if (!h || !queries || !stats) return fail(LQG_ERR_ARG, "null argument");
+    if (nq == 0 || nq > 0xFFFFFFFFull || k == 0 || ef == 0) return fail(LQG_ERR_ARG, "empty query set");
+    if (ef_class(ef) == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024");
+    lqg_status st = check_queries(queries, nq * h->dim);
+    if (st != LQG_OK) return st;
+    st = lqg::ensure_device(h);
+    if (st != LQG_OK) return st;
+    lqg::DevBuf<float> dq;
+    lqg::DevBuf<uint32_t> dids, dst;
+    HIP_TRY(dq.reserve(nq * h->dim));
+    HIP_TRY(dids.reserve(nq * k));
+    HIP_TRY(dst.reserve(2 * nq));
+    HIP_TRY(hipMemcpy(dq.p, queries, nq * h->dim * 4, hipMemcpyHostToDevice));
+    st = lqg::run_batch(h, dq.p, nq, nullptr, nullptr, ef, k, LQG_ORDER_NEAREST, dids.p, nullptr,
+                        dst.p, h->stream, true);
+    if (st != LQG_OK) return st;
+    HIP_TRY(hipMemcpyAsync(stats, dst.p, 2 * nq * 4, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->d_visit.release();
     return LQG_OK;
 }
 

@@ -79,6 +79,7 @@ struct lqg_index {
     lqg::DevBuf<float> d_q;
     lqg::DevBuf<uint32_t> d_ids, d_ef, d_k, d_stats, d_counter;
     lqg::DevBuf<float> d_dist;
+    lqg::DevBuf<uint32_t> d_visit;  // exact visit counting bitmaps
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // launch settings per pool-capacity class (64, 128, 256, 512, 1024)
     lqg::LaunchCfg launch[5];
@@ -107,7 +108,8 @@ lqg_status ensure_device(lqg_index* h);
 // Enqueues one search launch over device arrays.
 lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
                      const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, lqg_order order,
-                     uint32_t* d_results, float* d_dists, uint32_t* d_stats, hipStream_t s);
+                     uint32_t* d_results, float* d_dists, uint32_t* d_stats, hipStream_t s,
+                     bool count_visits = false);
 
 // Device graph build for lqg_build_index.
 lqg_status build_index(lqg_index* h, const float* data, uint32_t ef_build, uint32_t num_iter);

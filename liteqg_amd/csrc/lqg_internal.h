@@ -31,6 +31,11 @@ struct SearchJob {
     float* out_d;           // nq x kstride or null
     uint32_t* out_stats;    // nq x 2 (E, V) or null
     uint32_t* counter;      // persistent work queue head (device), zeroed before launch
+    // Exact visit counting (diagnostic launch only; null otherwise): one bitmap
+    // of visit_words 32-bit words per wave of the grid.  out_stats then holds
+    // (E, number of distinct nodes evaluated) instead of (E, evaluations).
+    uint32_t* visit_bits;
+    uint32_t visit_words;
 };
 
 // Launch geometry chosen by the host (autotuner or defaults).
@@ -54,5 +59,8 @@ hipError_t launch_reset_counter(uint32_t* counter, hipStream_t stream);
 
 // Number of workgroups that fill the device for this configuration.
 int search_full_grid(const DevIndex& ix, const LaunchCfg& cfg);
+
+// Workgroups launch_search will use for this job (grid after clamping).
+int search_groups(const DevIndex& ix, const SearchJob& job, const LaunchCfg& cfg);
 
 }  // namespace lqg

@@ -150,6 +150,13 @@ class Index:
         check(lib().lqg_autotune(self._h, ptr(Q), Q.shape[0], int(ef), int(k), C.byref(best)))
         return {f: getattr(best, f) for f, _ in Launch._fields_}
 
+    def count_visits(self, queries, ef: int, k: int = 10) -> np.ndarray:
+        """Per query (expansions, distinct nodes evaluated), counted exactly."""
+        Q = np.ascontiguousarray(queries, dtype=np.float32).reshape(-1, self.dimension)
+        out = np.empty((Q.shape[0], 2), dtype=np.uint32)
+        check(lib().lqg_count_visits(self._h, ptr(Q), Q.shape[0], int(ef), int(k), ptr(out)))
+        return out
+
     def info(self) -> dict:
         n, d, g = C.c_uint64(), C.c_uint64(), C.c_uint64()
         ep, ef = C.c_uint32(), C.c_uint32()

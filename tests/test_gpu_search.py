@@ -209,3 +209,32 @@ def test_integer_path_matches_oracle(oracle_mod, tmp_path, d):
             exp, exp_d, est = oracle_batch(oracle_mod, arr, Q, ef, 10, order)
             assert_same(ids, dis, exp, exp_d)
             assert st["expansions"] == int(est[:, 0].sum())
+
+
+@pytest.mark.parametrize("name", GOLDEN_INDEXES)
+def test_count_visits_equals_reference_visited_set(oracle_mod, name):
+    """E and the number of distinct nodes evaluated equal the oracle's exact
+    visited-set counts (the reference's own E and V on tie-free inputs)."""
+    arr, z = load_golden(name)
+    ix = device_index(name)
+    for ef in (10, 50, 100):
+        got = ix.count_visits(z["queries"], ef, 10).astype(np.int64)
+        _, _, est = oracle_batch(oracle_mod, arr, z["queries"], ef, 10, "nearest")
+        assert np.array_equal(got, est.astype(np.int64)), ef
+        if name in TIE_FREE:
+            _, _, _, (E, V) = oracle_mod.search_heap(arr, z["queries"][0], ef, 10)
+            assert (got[0, 0], got[0, 1]) == (E, V)
+
+
+def test_count_visits_with_tiny_cache(oracle_mod, tmp_path):
+    """With a 16-entry cache the kernel re-evaluates many nodes; the exact count
+    still equals the oracle's and is below the evaluation count."""
+    arr, base = make_random_index(6000, 64, 32, seed=21, knn=True)
+    ix = device_from_arrays(arr, tmp_path)
+    Q = base[:150] + 0.01
+    ix.set_launch(0, waves_per_group=4, cache_bits=4, occupancy=0)
+    got = ix.count_visits(Q, 100, 10).astype(np.int64)
+    _, _, est = oracle_batch(oracle_mod, arr, Q, 100, 10, "nearest")
+    assert np.array_equal(got, est.astype(np.int64))
+    _, st = ix.search_batch(Q, 10, ef=100, order="nearest", return_stats=True)
+    assert st["evaluations"] > int(got[:, 1].sum())
