@@ -234,3 +234,132 @@ def recall_at_k(found: np.ndarray, gt: np.ndarray, k: int) -> float:
     for f, g in zip(found, gt):
         tot += len(set(int(x) for x in f[:k]) & set(int(x) for x in g[:k]))
     return tot / (k * len(found))
+
+
+# --------------------------------------------------------------------------
+# H2: FastScan RaBitQ search (qg.hpp:252-287) over an explicit sidecar
+# --------------------------------------------------------------------------
+class _OrcH2(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("dim", C.c_uint32), ("deg", C.c_uint32), ("p", C.c_uint32),
+                ("ep", C.c_uint32), ("raw", C.c_void_p), ("nbr", C.c_void_p),
+                ("codes", C.c_void_p), ("factors", C.c_void_p), ("signs", C.c_void_p),
+                ("layout", C.c_int)]
+
+
+def padded_dim(d: int) -> int:
+    """P = 1 << ceil_log2(D) (utils/tools.hpp:25-34)."""
+    p = 1
+    while p < d:
+        p <<= 1
+    return p
+
+
+def _h2sigs():
+    L = lib()
+    if getattr(L, "_h2", False):
+        return L
+    P = C.c_void_p
+    L.orc_fht_rotate.argtypes = [P, P, C.c_uint32, C.c_uint32, P]
+    L.orc_query_prepare.argtypes = [P, P, C.c_uint32, C.c_uint32, P, P, P, P, P, P]
+    L.orc_scan_estimates.argtypes = [P, C.c_int, P, C.c_uint32, C.c_uint32, P, C.c_float,
+                                     C.c_float, C.c_float, C.c_int32, P, P]
+    L.orc_buffers_replay.argtypes = [P, C.c_uint64, C.c_uint32, P, P, P]
+    L.orc_search_h2.argtypes = [P, P, C.c_uint32, C.c_uint32, P, P, P]
+    L.orc_search_h2_batch.argtypes = [P, P, C.c_uint64, C.c_uint32, C.c_uint32, P, P, P, C.c_int]
+    L.orc_sidecar_build.argtypes = [P, P, P, C.c_int]
+    L.orc_codes_to_vector_major.argtypes = [P, C.c_uint32, C.c_uint32, P]
+    L._h2 = True
+    return L
+
+
+def fht_rotate(signs, x, p):
+    s = np.ascontiguousarray(signs, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty(p, np.float32)
+    _h2sigs().orc_fht_rotate(_p(s), _p(x), x.shape[0], p, _p(out))
+    return out
+
+
+def query_prepare(signs, q, p):
+    """(rotated P floats, lo, width, sumq, q bytes, LUT 4P bytes)."""
+    s = np.ascontiguousarray(signs, np.float32)
+    q = np.ascontiguousarray(q, np.float32)
+    rq = np.empty(p, np.float32)
+    lo = np.zeros(1, np.float32)
+    wd = np.zeros(1, np.float32)
+    sq = np.zeros(1, np.int32)
+    qb = np.empty(p, np.uint8)
+    lut = np.empty(4 * p, np.uint8)
+    _h2sigs().orc_query_prepare(_p(s), _p(q), q.shape[0], p, _p(rq), _p(lo), _p(wd), _p(sq),
+                                _p(qb), _p(lut))
+    return rq, float(lo[0]), float(wd[0]), int(sq[0]), qb, lut
+
+
+def scan_estimates(codes, layout, factors, deg, p, lut, sqr_y, lo, width, sumq):
+    codes = np.ascontiguousarray(codes, np.uint8)
+    factors = np.ascontiguousarray(factors, np.float32)
+    lut = np.ascontiguousarray(lut, np.uint8)
+    sums = np.empty(deg, np.uint16)
+    est = np.empty(deg, np.float32)
+    _h2sigs().orc_scan_estimates(_p(codes), layout, _p(factors), deg, p, _p(lut), sqr_y, lo,
+                                 width, sumq, _p(sums), _p(est))
+    return sums, est
+
+
+def buffers_replay(ops, cap):
+    ops = np.ascontiguousarray(ops, np.uint32)
+    nops = ops.shape[0] // 3
+    popped = np.empty(nops, np.uint32)
+    npop = np.zeros(1, np.uint32)
+    res = np.empty(cap, np.uint32)
+    _h2sigs().orc_buffers_replay(_p(ops), nops, cap, _p(popped), _p(npop), _p(res))
+    return popped[:int(npop[0])].copy(), res
+
+
+class H2Arrays:
+    """Everything the FastScan search reads: raw rows, ids, sidecar, signs."""
+
+    def __init__(self, ix: IndexArrays, signs, codes, factors, layout: int):
+        self.ix = ix
+        self.p = padded_dim(ix.dim)
+        self.signs = np.ascontiguousarray(signs, np.float32)
+        self.codes = np.ascontiguousarray(codes, np.uint8)
+        self.factors = np.ascontiguousarray(factors, np.float32)
+        self.layout = int(layout)
+        self._s = _OrcH2(ix.n, ix.dim, ix.deg, self.p, ix.ep, ix.raw.ctypes.data,
+                         ix.nbr.ctypes.data, self.codes.ctypes.data, self.factors.ctypes.data,
+                         self.signs.ctypes.data, self.layout)
+
+    @property
+    def ref(self):
+        return C.byref(self._s)
+
+
+def sidecar_build(ix: IndexArrays, signs, threads: int = 0):
+    """This build's sidecar (vector-major codes, FP64-summed factors)."""
+    p = padded_dim(ix.dim)
+    codes = np.zeros((ix.n, ix.deg * p // 8), np.uint8)
+    factors = np.zeros((ix.n, 3 * ix.deg), np.float32)
+    h = H2Arrays(ix, signs, codes, factors, 1)
+    _h2sigs().orc_sidecar_build(h.ref, _p(codes), _p(factors), threads)
+    return codes, factors
+
+
+def codes_to_vector_major(ref_codes, deg, p):
+    ref_codes = np.ascontiguousarray(ref_codes, np.uint8).reshape(-1, deg * p // 8)
+    out = np.empty_like(ref_codes)
+    L = _h2sigs()
+    for i in range(ref_codes.shape[0]):
+        L.orc_codes_to_vector_major(_p(ref_codes[i]), deg, p, _p(out[i]))
+    return out
+
+
+def search_h2_batch(h: H2Arrays, queries, ef: int, k: int, threads: int = 0):
+    Q = np.ascontiguousarray(queries, np.float32)
+    res = np.empty((Q.shape[0], k), np.uint32)
+    dis = np.empty((Q.shape[0], k), np.float32)
+    st = np.zeros((Q.shape[0], 2), np.uint64)
+    if _h2sigs().orc_search_h2_batch(h.ref, _p(Q), Q.shape[0], ef, k, _p(res), _p(dis), _p(st),
+                                     threads) != 0:
+        raise RuntimeError("orc_search_h2_batch failed")
+    return res, dis, st

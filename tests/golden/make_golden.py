@@ -31,6 +31,9 @@ sys.path.insert(0, ROOT)
 from liteqg_amd import datasets  # noqa: E402
 
 DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+DRIVER_H2 = os.path.join(ROOT, "oracle", "_ref", "ref_h2")
+H2_EFS = (10, 50, 100)
+H2_INDEXES = ("g_uniform_n2000_d64", "g_intmix_n1200_d128", "g_unitmix_n1000_d96_deg64")
 EFS = (10, 20, 50, 100)
 K = 10
 
@@ -75,6 +78,69 @@ def make_index(name, kind, n, d, deg, ef_build, nq, kw, seed):
                         efs=np.array(EFS, np.int64), **out)
 
 
+def padded(d):
+    p = 1
+    while p < d:
+        p <<= 1
+    return p
+
+
+def make_h2(name, nq=100, nprep=8, seed=3):
+    """Reference search_qg on the rebuilt row layout (oracle/ref_h2.cpp)."""
+    z = np.load(os.path.join(HERE, name + ".npz"))
+    n, d, deg = (int(x) for x in z["shape"])
+    p = padded(d)
+    rng = np.random.default_rng(seed + d)
+    signs = ((rng.integers(0, 2, p) * 2 - 1).astype(np.float32) /
+             np.sqrt(np.float32(p))).astype(np.float32)
+    q = np.ascontiguousarray(z["queries"][:nq])
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        sp, qp = os.path.join(tmp, "s.f32"), os.path.join(tmp, "q.f32")
+        signs.tofile(sp)
+        q.tofile(qp)
+        ipath = os.path.join(HERE, name + ".index")
+        for ef in H2_EFS:
+            o, sc, pr = (os.path.join(tmp, x) for x in ("o.u32", "side.bin", "prep.bin"))
+            subprocess.run([DRIVER_H2, "search", ipath, str(n), str(d), str(deg), sp, qp,
+                            str(nq), str(ef), str(K), o, sc, pr, str(nprep)], check=True,
+                           stdout=subprocess.DEVNULL)
+            out[f"ids_ef{ef}"] = np.fromfile(o, np.uint32).reshape(nq, K)
+        side = np.fromfile(sc, np.uint8).reshape(n, deg * p // 8 + 12 * deg)
+        prep = np.fromfile(pr, np.uint8).reshape(nprep, -1)
+    codes = side[:, :deg * p // 8].copy()
+    factors = side[:, deg * p // 8:].copy().view(np.float32).reshape(n, 3 * deg)
+    o = 0
+    rq = prep[:, o:o + 4 * p].copy().view(np.float32); o += 4 * p
+    lo = prep[:, o:o + 4].copy().view(np.float32)[:, 0]; o += 4
+    width = prep[:, o:o + 4].copy().view(np.float32)[:, 0]; o += 4
+    sumq = prep[:, o:o + 4].copy().view(np.int32)[:, 0]; o += 4
+    lut = prep[:, o:o + 4 * p].copy(); o += 4 * p
+    sqr_y = prep[:, o:o + 4].copy().view(np.float32)[:, 0]; o += 4
+    est = prep[:, o:o + 4 * deg].copy().view(np.float32)
+    np.savez_compressed(os.path.join(HERE, "h2_" + name + ".npz"), signs=signs, queries=q,
+                        codes=codes, factors=factors, prep_rq=rq, prep_lo=lo, prep_width=width,
+                        prep_sumq=sumq, prep_lut=lut, prep_sqr_y=sqr_y, prep_est=est,
+                        efs=np.array(H2_EFS, np.int64), k=np.int64(K), **out)
+
+
+def make_buffers(cap=16, nops=4000, seed=9):
+    """SearchBuffer / ResultBuffer operation traces (utils/buffer.hpp)."""
+    rng = np.random.default_rng(seed)
+    kind = rng.choice([0, 0, 0, 1, 2], size=nops).astype(np.uint32)
+    ids = rng.integers(0, 1 << 20, size=nops).astype(np.uint32)
+    dist = rng.integers(0, 40, size=nops).astype(np.float32)  # many equal distances
+    ops = np.stack([kind, ids, dist.view(np.uint32)], axis=1).reshape(-1)
+    with tempfile.TemporaryDirectory() as tmp:
+        op, outp = os.path.join(tmp, "ops.u32"), os.path.join(tmp, "out.bin")
+        ops.tofile(op)
+        subprocess.run([DRIVER_H2, "buffers", op, str(nops), str(cap), outp], check=True)
+        raw = np.fromfile(outp, np.uint32)
+    npop = int(raw[0])
+    np.savez_compressed(os.path.join(HERE, "buffers.npz"), ops=ops, cap=np.int64(cap),
+                        popped=raw[1:1 + npop], results=raw[1 + npop:1 + npop + cap])
+
+
 def make_funcs(d, m=None, seed=5):
     m = m or (64 if d <= 128 else 12)
     rng = np.random.default_rng(seed + d)
@@ -96,13 +162,19 @@ def make_funcs(d, m=None, seed=5):
 
 
 def main():
-    if not os.path.exists(DRIVER):
-        sys.exit("oracle/_ref/ref_driver missing: run `make -C oracle ref` first")
-    for d in FUNC_DIMS:
-        make_funcs(d)
-    for name, spec in INDEXES.items():
-        make_index(name, *spec)
-        print("wrote", name)
+    if not os.path.exists(DRIVER) or not os.path.exists(DRIVER_H2):
+        sys.exit("oracle/_ref drivers missing: run `make -C oracle ref` first")
+    if "--h2-only" not in sys.argv:
+        for d in FUNC_DIMS:
+            make_funcs(d)
+    if "--h2-only" not in sys.argv:
+        for name, spec in INDEXES.items():
+            make_index(name, *spec)
+            print("wrote", name)
+    make_buffers()
+    for name in H2_INDEXES:
+        make_h2(name)
+        print("wrote h2", name)
 
 
 if __name__ == "__main__":
