@@ -37,13 +37,27 @@ typedef enum {
     LQG_ORDER_NEAREST = 1    /* k nearest of the same pool, ascending            */
 } lqg_order;
 
+/* Search path used by lqg_search / lqg_search_batch / lqg_search_batch_device. */
+typedef enum {
+    /* QuantizedGraph::search_qg_with_heap (qg.hpp:298-344), the path the
+     * reference's Index.search runs (qg.hpp:235-243): SQ8 distances, dual heap */
+    LQG_MODE_SQ8 = 0,
+    /* QuantizedGraph::search_qg (qg.hpp:252-287): FastScan RaBitQ estimates
+     * steer a SearchBuffer(ef), exact L2 of expanded nodes fills a top-k
+     * ResultBuffer.  Needs a sidecar (lqg_h2_build_sidecar / _set_sidecar /
+     * _load_sidecar).  Output order is nearest first; `order` is ignored. */
+    LQG_MODE_FASTSCAN = 1
+} lqg_mode;
+
 typedef struct lqg_index lqg_index;
 
 /* Per-call counters (optional out-parameter of the batch entry points). */
 typedef struct {
     uint64_t queries;
     uint64_t expansions;  /* E: nodes popped and expanded                       */
-    uint64_t evaluations; /* distance evaluations done on the device            */
+    uint64_t evaluations; /* SQ8 mode: distance evaluations done on the device;
+                             FastScan mode: SearchBuffer pops (expanded or
+                             skipped as visited)                                */
     double device_ms;     /* kernel time measured with HIP events               */
 } lqg_stats;
 
@@ -98,7 +112,9 @@ lqg_status lqg_search_batch(lqg_index* index, const float* queries, uint64_t nq,
 
 /* Same as lqg_search_batch with every array in device memory, enqueued on
  * `stream` (a hipStream_t, NULL = the handle's stream) and not synchronised.
- * d_stats (optional) receives 2 x nq uint32 counters (E, V) per query. */
+ * d_stats (optional) receives 2 x nq uint32 counters per query: (E, V) in SQ8
+ * mode, (E, pops) in FastScan mode (E = 0xFFFFFFFF marks a query whose
+ * visited set overflowed; the host entry points fail instead). */
 lqg_status lqg_search_batch_device(lqg_index* index, const float* d_queries, uint64_t nq,
                                    const uint32_t* d_ef, const uint32_t* d_k, uint32_t ef_max,
                                    uint32_t kmax, lqg_order order, uint32_t* d_results,
@@ -134,6 +150,51 @@ lqg_status lqg_autotune(lqg_index* index, const float* queries, uint64_t nq, uin
  * (qg.hpp:320-328).  Used for algorithmic-byte accounting; not for speed. */
 lqg_status lqg_count_visits(lqg_index* index, const float* queries, uint64_t nq, uint32_t ef,
                             uint32_t k, uint32_t* stats);
+
+/* Selects the search path.  LQG_MODE_FASTSCAN requires a sidecar.  No
+ * counterpart in the reference, whose search() is hard-wired to the SQ8 path
+ * (qg.hpp:241-242). */
+lqg_status lqg_set_mode(lqg_index* index, lqg_mode mode);
+lqg_status lqg_get_mode(const lqg_index* index, lqg_mode* mode);
+
+/* FastScan sidecar: per node, RaBitQ codes (deg x P/8 bytes, P = padded
+ * dimension, 1 << ceil(log2 D), 64 <= P <= 2048) and factors (triple_x[deg],
+ * fac_dq[deg], fac_vq[deg]) of the node's neighbours relative to the node,
+ * under the rotation x -> H_P diag(signs) [x, 0]; the row layout the reference
+ * documents (qg.hpp:73-78) and fills with rabitq_codes (rabitq.hpp:23-59),
+ * which LiteQG itself never calls (qg.hpp:489-501).
+ *
+ * Builds it on the device from the loaded index: signs = +-1/sqrt(P) (as
+ * FHTRotator draws them, utils/rotator.hpp:37-43) from `signs` (P floats) if
+ * given, else from a generator seeded with `seed`. */
+lqg_status lqg_h2_build_sidecar(lqg_index* index, const float* signs, uint64_t seed);
+
+/* Installs a sidecar from host arrays: signs (P), codes (N x deg*P/8 bytes) in
+ * the reference's FastScan block layout (layout 0, fastscan_impl.hpp:67-124)
+ * or vector-major (layout 1: neighbour j's P/8 bytes, byte b = codebook 2b in
+ * the low nibble, 2b+1 in the high nibble, bit 3 = first dim), factors
+ * (N x 3*deg floats). */
+lqg_status lqg_h2_set_sidecar(lqg_index* index, const float* signs, const uint8_t* codes,
+                              int layout, const float* factors);
+
+/* Copies the sidecar out (codes vector-major); any pointer may be NULL. */
+lqg_status lqg_h2_get_sidecar(const lqg_index* index, float* signs, uint8_t* codes,
+                              float* factors);
+
+/* Sidecar file: header (magic, N, D, deg, P, fingerprint of the index's
+ * vectors and neighbour ids), signs, codes (vector-major), factors.  Loading
+ * checks the header against the loaded index. */
+lqg_status lqg_h2_save_sidecar(const lqg_index* index, const char* path);
+lqg_status lqg_h2_load_sidecar(lqg_index* index, const char* path);
+
+/* Parity probe of the FastScan path: for each of nq queries (host memory) the
+ * device's query preparation (QGQuery::query_prepare, qg_query.hpp:28-43):
+ * lut (nq x 4P bytes, the reference's layout), lo_width (nq x 2), sumq (nq);
+ * and if nodes (nq ids) is given, the exact distance to nodes[i] (sqr_y, nq)
+ * and the FastScan estimates of its deg neighbours (est, nq x deg). */
+lqg_status lqg_h2_probe(lqg_index* index, const float* queries, uint64_t nq,
+                        const uint32_t* nodes, uint8_t* lut, float* lo_width, int32_t* sumq,
+                        float* sqr_y, float* est);
 
 /* Read-only views of the loaded index (host copies). */
 lqg_status lqg_info(const lqg_index* index, uint64_t* num_elements, uint64_t* dimension,

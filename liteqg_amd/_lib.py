@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "liblqg.so")
 
 LQG_OK = 0
 ORDER = {"reference": 0, "nearest": 1}
+MODE = {"sq8": 0, "fastscan": 1}
 
 
 class LqgError(RuntimeError):
@@ -55,6 +56,14 @@ _SIGS = {
     "lqg_count_visits": [_P, _P, C.c_uint64, C.c_uint32, C.c_uint32, _P],
     "lqg_info": [_P, _P, _P, _P, _P, _P],
     "lqg_get_graph": [_P, _P],
+    "lqg_set_mode": [_P, C.c_int],
+    "lqg_get_mode": [_P, _P],
+    "lqg_h2_build_sidecar": [_P, _P, C.c_uint64],
+    "lqg_h2_set_sidecar": [_P, _P, _P, C.c_int, _P],
+    "lqg_h2_get_sidecar": [_P, _P, _P, _P],
+    "lqg_h2_save_sidecar": [_P, C.c_char_p],
+    "lqg_h2_load_sidecar": [_P, C.c_char_p],
+    "lqg_h2_probe": [_P, _P, C.c_uint64, _P, _P, _P, _P, _P, _P],
 }
 EXPORTED = tuple(_SIGS) + ("lqg_last_error",)
 

@@ -38,20 +38,40 @@ def sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
 
 
-def build(verbose: bool = False, extra=()) -> str:
-    """Compile every source into liblqg.so (always recompiles)."""
-    if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
-        raise RuntimeError(f"hipcc not found at {HIPCC}")
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, *extra, "-o", tmp, *sources()]
+def _compile(src: str, obj: str, extra, verbose: bool) -> None:
+    flags = [f for f in FLAGS if f != "-shared"]
+    cmd = [HIPCC, *flags, *extra, "-c", "-o", obj, src]
     if verbose:
         print(" ".join(cmd))
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed")
-    if verbose and r.stderr.strip():
-        sys.stderr.write(r.stderr)
+        raise RuntimeError(f"hipcc failed on {os.path.basename(src)}")
+
+
+def build(verbose: bool = False, extra=()) -> str:
+    """Compile every source into liblqg.so (always recompiles; one hipcc
+    process per source, in parallel, then one link)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
+        raise RuntimeError(f"hipcc not found at {HIPCC}")
+    objdir = os.path.join(HERE, "_obj")
+    os.makedirs(objdir, exist_ok=True)
+    srcs = sources()
+    objs = [os.path.join(objdir, os.path.basename(s) + ".o") for s in srcs]
+    jobs = max(1, min(len(srcs), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(jobs) as ex:
+        for f in [ex.submit(_compile, s, o, extra, verbose) for s, o in zip(srcs, objs)]:
+            f.result()
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    if verbose:
+        print(" ".join(cmd))
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc link failed")
     os.replace(tmp, LIB)
     return LIB
 

@@ -40,20 +40,10 @@
 #include <math.h>
 
 #include "lqg_internal.h"
+#include "wave.h"
 
 namespace lqg {
 namespace {
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
-    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
-}
 
 __device__ __forceinline__ uint64_t make_key(float d, uint32_t id) {
     return (static_cast<uint64_t>(__float_as_uint(d)) << 32) | (static_cast<uint64_t>(id) << 1);
@@ -63,13 +53,6 @@ __device__ __forceinline__ uint32_t key_id(uint64_t k) {
 }
 __device__ __forceinline__ float key_dist(uint64_t k) {
     return __uint_as_float(static_cast<uint32_t>(k >> 32));
-}
-
-__device__ __forceinline__ uint64_t readlane64(uint64_t v, int l) {
-    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v)), l);
-    const uint32_t hi =
-        __builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(v >> 32)), l);
-    return (static_cast<uint64_t>(hi) << 32) | lo;
 }
 
 // One SQ8 byte into partial s: x = min + q*scale (one FMA, as the reference

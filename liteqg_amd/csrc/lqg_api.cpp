@@ -129,6 +129,11 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
                      const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, lqg_order order,
                      uint32_t* d_results, float* d_dists, uint32_t* d_stats, hipStream_t s,
                      bool count_visits) {
+    if (h->mode == LQG_MODE_FASTSCAN) {
+        if (count_visits) return fail(LQG_ERR_UNSUPPORTED, "visit counting is for the SQ8 path");
+        return run_h2_batch(h, d_queries, nq, d_ef, d_k, ef_max, kmax, d_results, d_dists, d_stats,
+                            s);
+    }
     if (ef_class(ef_max) == 0)
         return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported on the device");
     SearchJob job;
@@ -224,6 +229,10 @@ lqg_status lqg_destroy(lqg_index* h) {
     h->d_counter.release();
     h->d_dist.release();
     h->d_visit.release();
+    h->d_h2rec.release();
+    h->d_h2signs.release();
+    h->d_h2gvis.release();
+    h->d_h2err.release();
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -295,6 +304,7 @@ lqg_status lqg_load(lqg_index* h, const char* filename) {
     h->sq8.swap(sq8);
     h->have_data = true;
     h->on_device = false;
+    lqg::h2_reset(h);
     return LQG_OK;
 }
 
@@ -349,7 +359,7 @@ lqg_status lqg_search(lqg_index* h, const float* query, uint32_t k, uint32_t* re
     if (st != LQG_OK) return st;
     HIP_TRY(hipMemcpyAsync(results, h->d_ids.p, k * 4ull, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    return LQG_OK;
+    return lqg::h2_check_error(h);
 }
 
 lqg_status lqg_search_batch(lqg_index* h, const float* queries, uint64_t nq, const uint32_t* ef,
@@ -401,6 +411,8 @@ lqg_status lqg_search_batch(lqg_index* h, const float* queries, uint64_t nq, con
                                h->stream));
     }
     HIP_TRY(hipStreamSynchronize(h->stream));
+    st = lqg::h2_check_error(h);
+    if (st != LQG_OK) return st;
     if (stats) {
         stats->queries = nq;
         stats->expansions = 0;
@@ -444,9 +456,25 @@ lqg_status lqg_set_launch(lqg_index* h, uint32_t ef, const lqg_launch* cfg) {
                 cfg->cache_bits > 14 || (cfg->occupancy != 0 && cfg->occupancy != 8) ||
                 cfg->groups < 0))
         return fail(LQG_ERR_ARG, "launch settings out of range");
+    const bool fs = h->mode == LQG_MODE_FASTSCAN;
+    if (fs && cfg && cfg->cache_bits < 6)
+        return fail(LQG_ERR_ARG, "FastScan visited-table bits must be in [6, 14]");
     for (int slot = 0; slot < 5; ++slot) {
         const int cap = 64 << slot;
         if (ef != 0 && ef_class(ef) != cap) continue;
+        if (fs) {
+            if (!cfg) {
+                h->h2_launch_set[slot] = false;
+                continue;
+            }
+            lqg::H2Cfg c = lqg::h2_default_launch(cap);
+            c.waves_per_group = cfg->waves_per_group;
+            c.hash_bits = cfg->cache_bits;
+            c.groups = cfg->groups;
+            h->h2_launch[slot] = c;
+            h->h2_launch_set[slot] = true;
+            continue;
+        }
         if (!cfg) {
             h->launch_set[slot] = false;
             continue;
@@ -468,6 +496,15 @@ lqg_status lqg_get_launch(const lqg_index* h, uint32_t ef, lqg_launch* cfg) {
     const int cls = ef_class(ef == 0 ? 1 : ef);
     if (cls == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024");
     const int slot = lqg::class_slot(cls);
+    if (h->mode == LQG_MODE_FASTSCAN) {
+        const lqg::H2Cfg c =
+            h->h2_launch_set[slot] ? h->h2_launch[slot] : lqg::h2_default_launch(cls);
+        cfg->waves_per_group = c.waves_per_group;
+        cfg->cache_bits = c.hash_bits;
+        cfg->occupancy = 0;
+        cfg->groups = c.groups;
+        return LQG_OK;
+    }
     const lqg::LaunchCfg c = h->launch_set[slot] ? h->launch[slot] : lqg::default_launch(cls);
     cfg->waves_per_group = c.waves_per_group;
     cfg->cache_bits = c.cache_bits;
@@ -492,6 +529,57 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
     HIP_TRY(dids.reserve(nq * k));
     HIP_TRY(hipMemcpy(dq.p, queries, nq * h->dim * 4, hipMemcpyHostToDevice));
     const int slot = lqg::class_slot(cls);
+    if (h->mode == LQG_MODE_FASTSCAN) {
+        // waves per workgroup x LDS visited-table size
+        const bool had = h->h2_launch_set[slot];
+        const lqg::H2Cfg saved = h->h2_launch[slot];
+        const lqg::H2Cfg base = lqg::h2_default_launch(cls);
+        float best_ms = 1e30f;
+        lqg::H2Cfg best_cfg = base;
+        const int wpg[] = {1, 2, 4};
+        for (int extra = 0; extra < 2; ++extra)
+            for (int wg : wpg) {
+                lqg::H2Cfg c = base;
+                c.waves_per_group = wg;
+                c.hash_bits = base.hash_bits + extra;
+                h->h2_launch[slot] = c;
+                h->h2_launch_set[slot] = true;
+                float ms_sum = 0.f;
+                bool ok = true;
+                for (int rep = 0; rep < 3 && ok; ++rep) {
+                    if (hipEventRecord(h->ev0, h->stream) != hipSuccess) ok = false;
+                    if (ok && lqg::run_batch(h, dq.p, nq, nullptr, nullptr, ef, k, LQG_ORDER_NEAREST,
+                                             dids.p, nullptr, nullptr, h->stream) != LQG_OK)
+                        ok = false;
+                    if (ok && hipEventRecord(h->ev1, h->stream) != hipSuccess) ok = false;
+                    if (ok && hipEventSynchronize(h->ev1) != hipSuccess) ok = false;
+                    float ms = 0.f;
+                    if (ok && hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess) ok = false;
+                    if (rep > 0) ms_sum += ms;
+                }
+                (void)hipGetLastError();
+                if (ok && ms_sum / 2 < best_ms) {
+                    best_ms = ms_sum / 2;
+                    best_cfg = c;
+                }
+            }
+        dq.release();
+        dids.release();
+        if (best_ms >= 1e30f) {
+            h->h2_launch[slot] = saved;
+            h->h2_launch_set[slot] = had;
+            return fail(LQG_ERR_DEVICE, "no launch setting ran");
+        }
+        h->h2_launch[slot] = best_cfg;
+        h->h2_launch_set[slot] = true;
+        if (best) {
+            best->waves_per_group = best_cfg.waves_per_group;
+            best->cache_bits = best_cfg.hash_bits;
+            best->occupancy = 0;
+            best->groups = best_cfg.groups;
+        }
+        return LQG_OK;
+    }
     const bool had = h->launch_set[slot];
     const lqg::LaunchCfg saved = h->launch[slot];
     float best_ms = 1e30f;
@@ -602,7 +690,9 @@ lqg_status lqg_build_index(lqg_index* h, const float* data, uint64_t num_points,
     if (h->n <= h->deg) return fail(LQG_ERR_ARG, "num_elements must exceed degree_bound");
     if (check_queries(data, num_points * dimension) != LQG_OK)
         return fail(LQG_ERR_ARG, "data contains a non-finite value");
-    return lqg::build_index(h, data, ef_build, num_iter);
+    lqg_status st = lqg::build_index(h, data, ef_build, num_iter);
+    if (st == LQG_OK) lqg::h2_reset(h);
+    return st;
 }
 
 }  // extern "C"

@@ -1,0 +1,446 @@
+// lqg_h2.cpp -- host side of the FastScan RaBitQ path (search_qg,
+// qg.hpp:252-287): the node-record layout, sidecar construction on the
+// device, installation from host arrays (either code layout), persistence,
+// the parity probe and the batch launcher.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "lqg_index.h"
+
+namespace lqg {
+
+namespace {
+
+// Per-wave overflow visited table in device memory: 2^14 entries (64 KiB),
+// used once a query has expanded more nodes than half its LDS table holds.
+constexpr uint32_t kGvisBits = 14;
+
+uint32_t padded_dim(uint64_t d) {
+    uint32_t p = 1;
+    while (p < d) p <<= 1;
+    return p;
+}
+
+uint32_t align_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
+
+// Record layout for the loaded index; fails for shapes the kernels do not
+// take.  P = 1 << ceil(log2 D) (utils/tools.hpp:25-34); FHTRotator handles
+// P = 2^6 .. 2^11 (utils/rotator.hpp:54-77).
+lqg_status set_layout(lqg_index* h) {
+    const uint32_t p = padded_dim(h->dim);
+    if (p < 64 || p > 2048)
+        return fail(LQG_ERR_UNSUPPORTED,
+                    "FastScan needs a padded dimension in [64, 2048] (33 <= D <= 2048)");
+    const uint32_t lpv = h->deg >= 64 ? 1u : 2u;
+    if (p / 8 / lpv / 4 > 32)
+        return fail(LQG_ERR_UNSUPPORTED, "FastScan: P = 2048 with degree >= 64 is not supported");
+    const uint32_t deg = static_cast<uint32_t>(h->deg), dim = static_cast<uint32_t>(h->dim);
+    h->h2_p = p;
+    h->h2_off_codes = align_up(4 * dim, 16);
+    h->h2_off_fac = h->h2_off_codes + deg * (p / 8);
+    h->h2_off_ids = h->h2_off_fac + 12 * deg;
+    h->h2_stride = align_up(h->h2_off_ids + 4 * deg, 128);
+    return LQG_OK;
+}
+
+// Records with the raw rows and neighbour ids of the host copy in place
+// (codes and factors still to be written).
+lqg_status alloc_records(lqg_index* h) {
+    lqg_status st = set_layout(h);
+    if (st != LQG_OK) return st;
+    h->h2_ready = false;
+    LQG_HIP_TRY(h->d_h2rec.reserve(h->n * static_cast<size_t>(h->h2_stride)));
+    LQG_HIP_TRY(h->d_h2signs.reserve(h->h2_p));
+    LQG_HIP_TRY(hipMemcpy2DAsync(h->d_h2rec.p, h->h2_stride, h->raw.data(), 4 * h->dim, 4 * h->dim,
+                                 h->n, hipMemcpyHostToDevice, h->stream));
+    LQG_HIP_TRY(hipMemcpy2DAsync(h->d_h2rec.p + h->h2_off_ids, h->h2_stride, h->nbr.data(),
+                                 4 * h->deg, 4 * h->deg, h->n, hipMemcpyHostToDevice, h->stream));
+    LQG_HIP_TRY(hipMemcpyAsync(h->d_h2signs.p, h->h2_signs.data(), 4ull * h->h2_p,
+                               hipMemcpyHostToDevice, h->stream));
+    LQG_HIP_TRY(hipStreamSynchronize(h->stream));
+    return LQG_OK;
+}
+
+uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Nibble of neighbour v for codebook c in the reference's FastScan block
+// layout (pack_codes / pack_codes_helper, fastscan_impl.hpp:67-124): blocks
+// of 32 neighbours, 4P bytes each; inside a block, 32 bytes per pair of
+// codebooks, the first 16 for the even codebook and the next 16 for the odd
+// one; byte j of those 16 holds neighbour perm[j] in its low nibble and
+// perm[j] + 16 in its high nibble, perm = 0, 8, 1, 9, ..., 7, 15.
+uint32_t block_nibble(const uint8_t* codes, uint32_t p, uint32_t v, uint32_t c) {
+    const uint32_t w = v & 31u, u = w & 15u;
+    const uint32_t j = u < 8 ? 2 * u : 2 * (u - 8) + 1;
+    const uint8_t b = codes[static_cast<size_t>(v >> 5) * 4 * p + (c >> 1) * 32 + (c & 1u) * 16 + j];
+    return w < 16 ? (b & 15u) : (b >> 4);
+}
+
+// FNV-1a over 32-bit words of the entry point, neighbour ids and vectors:
+// ties a sidecar file to the index it was built from.
+uint64_t fingerprint(const lqg_index* h) {
+    uint64_t x = 0xCBF29CE484222325ull;
+    auto mix = [&x](uint32_t w) {
+        x ^= w;
+        x *= 0x100000001B3ull;
+    };
+    mix(h->ep);
+    for (uint32_t v : h->nbr) mix(v);
+    for (float f : h->raw) {
+        uint32_t w;
+        memcpy(&w, &f, 4);
+        mix(w);
+    }
+    return x;
+}
+
+struct SideHeader {
+    char magic[8];
+    uint32_t version, n, dim, deg, p, reserved;
+    uint64_t fingerprint;
+};
+constexpr char kMagic[8] = {'L', 'Q', 'G', 'S', 'I', 'D', 'E', '1'};
+
+lqg_status need_data(lqg_index* h) {
+    if (!h->have_data)
+        return fail(LQG_ERR_STATE, "index has no data: call load() or build_index() first");
+    return ensure_device(h);
+}
+
+}  // namespace
+
+H2Cfg h2_default_launch(int ef_cap) {
+    H2Cfg c;
+    c.ef_cap = ef_cap;
+    c.waves_per_group = 4;
+    c.groups = 0;
+    // LDS visited table: at least 4x the largest ef of the class, so that
+    // expansions (typically 1x-3x ef) stay under half of it
+    c.hash_bits = ef_cap <= 64 ? 9 : ef_cap <= 256 ? 10 : ef_cap <= 512 ? 11 : 12;
+    return c;
+}
+
+void h2_reset(lqg_index* h) {
+    h->d_h2rec.release();
+    h->h2_ready = false;
+    h->mode = LQG_MODE_SQ8;
+}
+
+lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
+                        const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, uint32_t* d_results,
+                        float* d_dists, uint32_t* d_stats, hipStream_t s) {
+    if (!h->h2_ready) return fail(LQG_ERR_STATE, "FastScan mode needs a sidecar");
+    const int cls = ef_class(ef_max);
+    if (cls == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported on the device");
+    const int slot = class_slot(cls);
+    H2Cfg cfg = h->h2_launch_set[slot] ? h->h2_launch[slot] : h2_default_launch(cls);
+    cfg.ef_cap = cls;
+    const H2Dev ix = h->h2dev();
+    if (nq < static_cast<uint64_t>(cfg.waves_per_group)) cfg.waves_per_group = 1;
+    while (cfg.waves_per_group > 1 && h2_lds_bytes(ix, cfg, kmax) > 160 * 1024)
+        cfg.waves_per_group /= 2;
+    if (h2_lds_bytes(ix, cfg, kmax) > 160 * 1024)
+        return fail(LQG_ERR_UNSUPPORTED, "FastScan: kmax too large for the LDS of one workgroup");
+    const int groups = h2_groups(ix, cfg, kmax, static_cast<uint32_t>(nq));
+    if (groups <= 0) return fail(LQG_ERR_DEVICE, "FastScan: no launch geometry");
+    cfg.groups = groups;
+    LQG_HIP_TRY(h->d_h2gvis.reserve((static_cast<size_t>(groups) * cfg.waves_per_group) << kGvisBits));
+    LQG_HIP_TRY(h->d_h2err.reserve(1));
+    LQG_HIP_TRY(h->d_counter.reserve(1));
+    H2Job job;
+    job.queries = d_queries;
+    job.ef = d_ef;
+    job.k = d_k;
+    job.nq = static_cast<uint32_t>(nq);
+    job.ef_all = ef_max;
+    job.k_all = kmax;
+    job.kstride = kmax;
+    job.out_ids = d_results;
+    job.out_d = d_dists;
+    job.out_stats = d_stats;
+    job.counter = h->d_counter.p;
+    job.gvis = h->d_h2gvis.p;
+    job.gvis_bits = kGvisBits;
+    job.err = h->d_h2err.p;
+    LQG_HIP_TRY(launch_reset_counter(h->d_counter.p, s));
+    LQG_HIP_TRY(launch_reset_counter(h->d_h2err.p, s));
+    LQG_HIP_TRY(h2_launch_search(ix, job, cfg, s));
+    return LQG_OK;
+}
+
+lqg_status h2_check_error(lqg_index* h) {
+    if (h->mode != LQG_MODE_FASTSCAN || !h->d_h2err.p) return LQG_OK;
+    uint32_t e = 0;
+    LQG_HIP_TRY(hipMemcpy(&e, h->d_h2err.p, 4, hipMemcpyDeviceToHost));
+    if (e) return fail(LQG_ERR_UNSUPPORTED, "FastScan: a query's visited set overflowed");
+    return LQG_OK;
+}
+
+}  // namespace lqg
+
+using lqg::fail;
+
+extern "C" {
+
+lqg_status lqg_set_mode(lqg_index* h, lqg_mode mode) {
+    if (!h) return fail(LQG_ERR_ARG, "null index");
+    if (mode == LQG_MODE_SQ8) {
+        h->mode = mode;
+        return LQG_OK;
+    }
+    if (mode != LQG_MODE_FASTSCAN) return fail(LQG_ERR_ARG, "unknown mode");
+    if (!h->h2_ready)
+        return fail(LQG_ERR_STATE, "FastScan mode needs a sidecar: build, set or load one first");
+    h->mode = mode;
+    return LQG_OK;
+}
+
+lqg_status lqg_get_mode(const lqg_index* h, lqg_mode* mode) {
+    if (!h || !mode) return fail(LQG_ERR_ARG, "null argument");
+    *mode = h->mode;
+    return LQG_OK;
+}
+
+lqg_status lqg_h2_build_sidecar(lqg_index* h, const float* signs, uint64_t seed) {
+    if (!h) return fail(LQG_ERR_ARG, "null index");
+    lqg_status st = lqg::need_data(h);
+    if (st != LQG_OK) return st;
+    st = lqg::set_layout(h);
+    if (st != LQG_OK) return st;
+    const uint32_t p = h->h2_p;
+    std::vector<float> sg(p);
+    if (signs) {
+        for (uint32_t i = 0; i < p; ++i) {
+            if (!(fabsf(signs[i]) > 0.0f) || !std::isfinite(signs[i]))
+                return fail(LQG_ERR_ARG, "signs must be finite and non-zero");
+            sg[i] = signs[i];
+        }
+    } else {
+        // FHTRotator draws (2 * bernoulli - 1) / sqrt(P) (utils/rotator.hpp:37-43)
+        uint64_t s = seed;
+        uint64_t bits = 0;
+        for (uint32_t i = 0; i < p; ++i) {
+            if (i % 64 == 0) bits = lqg::splitmix64(s);
+            const int b = static_cast<int>((bits >> (i % 64)) & 1u);
+            sg[i] = static_cast<float>(2 * b - 1) / sqrtf(static_cast<float>(p));
+        }
+    }
+    h->h2_signs = sg;
+    st = lqg::alloc_records(h);
+    if (st != LQG_OK) return st;
+    lqg::DevBuf<float> rot;
+    LQG_HIP_TRY(rot.reserve(h->n * static_cast<size_t>(p)));
+    hipError_t e = lqg::h2_launch_sidecar(h->h2dev(), rot.p, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    rot.release();
+    if (e != hipSuccess) return lqg::hip_fail(e, "sidecar build");
+    h->h2_ready = true;
+    return LQG_OK;
+}
+
+lqg_status lqg_h2_set_sidecar(lqg_index* h, const float* signs, const uint8_t* codes, int layout,
+                              const float* factors) {
+    if (!h || !signs || !codes || !factors) return fail(LQG_ERR_ARG, "null argument");
+    if (layout != 0 && layout != 1) return fail(LQG_ERR_ARG, "layout must be 0 or 1");
+    lqg_status st = lqg::need_data(h);
+    if (st != LQG_OK) return st;
+    st = lqg::set_layout(h);
+    if (st != LQG_OK) return st;
+    const uint32_t p = h->h2_p, deg = static_cast<uint32_t>(h->deg);
+    const size_t cb = static_cast<size_t>(deg) * (p / 8);
+    h->h2_signs.assign(signs, signs + p);
+    st = lqg::alloc_records(h);
+    if (st != LQG_OK) return st;
+    if (layout == 1) {
+        LQG_HIP_TRY(hipMemcpy2D(h->d_h2rec.p + h->h2_off_codes, h->h2_stride, codes, cb, cb, h->n,
+                                hipMemcpyHostToDevice));
+    } else {
+        // block layout -> vector-major, a chunk of nodes at a time
+        const uint64_t chunk = std::max<uint64_t>(1, (32ull << 20) / cb);
+        std::vector<uint8_t> vm(std::min<uint64_t>(chunk, h->n) * cb);
+        for (uint64_t s = 0; s < h->n; s += chunk) {
+            const uint64_t e = std::min<uint64_t>(h->n, s + chunk);
+            std::fill(vm.begin(), vm.end(), 0);
+            for (uint64_t u = s; u < e; ++u) {
+                const uint8_t* src = codes + u * cb;
+                uint8_t* dst = vm.data() + (u - s) * cb;
+                for (uint32_t v = 0; v < deg; ++v)
+                    for (uint32_t c = 0; c < p / 4; ++c) {
+                        const uint32_t nib = lqg::block_nibble(src, p, v, c);
+                        dst[static_cast<size_t>(v) * (p / 8) + c / 2] |=
+                            static_cast<uint8_t>((c & 1u) ? nib << 4 : nib);
+                    }
+            }
+            LQG_HIP_TRY(hipMemcpy2D(h->d_h2rec.p + s * h->h2_stride + h->h2_off_codes,
+                                    h->h2_stride, vm.data(), cb, cb, e - s,
+                                    hipMemcpyHostToDevice));
+        }
+    }
+    LQG_HIP_TRY(hipMemcpy2D(h->d_h2rec.p + h->h2_off_fac, h->h2_stride, factors, 12ull * deg,
+                            12ull * deg, h->n, hipMemcpyHostToDevice));
+    h->h2_ready = true;
+    return LQG_OK;
+}
+
+lqg_status lqg_h2_get_sidecar(const lqg_index* h, float* signs, uint8_t* codes, float* factors) {
+    if (!h) return fail(LQG_ERR_ARG, "null index");
+    if (!h->h2_ready) return fail(LQG_ERR_STATE, "no sidecar");
+    const uint32_t p = h->h2_p, deg = static_cast<uint32_t>(h->deg);
+    const size_t cb = static_cast<size_t>(deg) * (p / 8);
+    if (signs) memcpy(signs, h->h2_signs.data(), 4ull * p);
+    if (codes)
+        LQG_HIP_TRY(hipMemcpy2D(codes, cb, h->d_h2rec.p + h->h2_off_codes, h->h2_stride, cb, h->n,
+                                hipMemcpyDeviceToHost));
+    if (factors)
+        LQG_HIP_TRY(hipMemcpy2D(factors, 12ull * deg, h->d_h2rec.p + h->h2_off_fac, h->h2_stride,
+                                12ull * deg, h->n, hipMemcpyDeviceToHost));
+    return LQG_OK;
+}
+
+lqg_status lqg_h2_save_sidecar(const lqg_index* h, const char* path) {
+    if (!h || !path) return fail(LQG_ERR_ARG, "null argument");
+    if (!h->h2_ready) return fail(LQG_ERR_STATE, "no sidecar");
+    const uint32_t p = h->h2_p, deg = static_cast<uint32_t>(h->deg);
+    const size_t cb = static_cast<size_t>(deg) * (p / 8), fb = 12ull * deg;
+    FILE* f = fopen(path, "wb");
+    if (!f) return fail(LQG_ERR_IO, std::string("cannot create ") + path);
+    lqg::SideHeader hd;
+    memcpy(hd.magic, lqg::kMagic, 8);
+    hd.version = 1;
+    hd.n = static_cast<uint32_t>(h->n);
+    hd.dim = static_cast<uint32_t>(h->dim);
+    hd.deg = deg;
+    hd.p = p;
+    hd.reserved = 0;
+    hd.fingerprint = lqg::fingerprint(h);
+    bool ok = fwrite(&hd, sizeof hd, 1, f) == 1;
+    ok = ok && fwrite(h->h2_signs.data(), 4, p, f) == p;
+    const uint64_t chunk = std::max<uint64_t>(1, (64ull << 20) / (cb + fb));
+    std::vector<uint8_t> c(std::min<uint64_t>(chunk, h->n) * cb);
+    std::vector<uint8_t> fc(std::min<uint64_t>(chunk, h->n) * fb);
+    // codes of every node, then factors of every node
+    for (int part = 0; ok && part < 2; ++part) {
+        const size_t w = part == 0 ? cb : fb;
+        const uint32_t off = part == 0 ? h->h2_off_codes : h->h2_off_fac;
+        std::vector<uint8_t>& buf = part == 0 ? c : fc;
+        for (uint64_t s = 0; ok && s < h->n; s += chunk) {
+            const uint64_t e = std::min<uint64_t>(h->n, s + chunk);
+            if (hipMemcpy2D(buf.data(), w, h->d_h2rec.p + s * h->h2_stride + off, h->h2_stride, w,
+                            e - s, hipMemcpyDeviceToHost) != hipSuccess) {
+                fclose(f);
+                return fail(LQG_ERR_DEVICE, "sidecar download failed");
+            }
+            ok = fwrite(buf.data(), 1, (e - s) * w, f) == (e - s) * w;
+        }
+    }
+    ok = (fclose(f) == 0) && ok;
+    return ok ? LQG_OK : fail(LQG_ERR_IO, std::string("write failed: ") + path);
+}
+
+lqg_status lqg_h2_load_sidecar(lqg_index* h, const char* path) {
+    if (!h || !path) return fail(LQG_ERR_ARG, "null argument");
+    lqg_status st = lqg::need_data(h);
+    if (st != LQG_OK) return st;
+    st = lqg::set_layout(h);
+    if (st != LQG_OK) return st;
+    const uint32_t p = h->h2_p, deg = static_cast<uint32_t>(h->deg);
+    const size_t cb = static_cast<size_t>(deg) * (p / 8), fb = 12ull * deg;
+    FILE* f = fopen(path, "rb");
+    if (!f) return fail(LQG_ERR_IO, std::string("cannot open ") + path);
+    lqg::SideHeader hd;
+    bool ok = fread(&hd, sizeof hd, 1, f) == 1;
+    if (!ok || memcmp(hd.magic, lqg::kMagic, 8) != 0 || hd.version != 1) {
+        fclose(f);
+        return fail(LQG_ERR_IO, std::string(path) + " is not a sidecar file");
+    }
+    if (hd.n != h->n || hd.dim != h->dim || hd.deg != deg || hd.p != p ||
+        hd.fingerprint != lqg::fingerprint(h)) {
+        fclose(f);
+        return fail(LQG_ERR_IO, std::string(path) + " was built for another index");
+    }
+    std::vector<float> sg(p);
+    ok = fread(sg.data(), 4, p, f) == p;
+    h->h2_signs = sg;
+    if (ok) {
+        st = lqg::alloc_records(h);
+        if (st != LQG_OK) {
+            fclose(f);
+            return st;
+        }
+    }
+    const uint64_t chunk = std::max<uint64_t>(1, (64ull << 20) / (cb + fb));
+    std::vector<uint8_t> buf(std::min<uint64_t>(chunk, h->n) * std::max(cb, fb));
+    for (int part = 0; ok && part < 2; ++part) {
+        const size_t w = part == 0 ? cb : fb;
+        const uint32_t off = part == 0 ? h->h2_off_codes : h->h2_off_fac;
+        for (uint64_t s = 0; ok && s < h->n; s += chunk) {
+            const uint64_t e = std::min<uint64_t>(h->n, s + chunk);
+            ok = fread(buf.data(), 1, (e - s) * w, f) == (e - s) * w;
+            if (ok && hipMemcpy2D(h->d_h2rec.p + s * h->h2_stride + off, h->h2_stride, buf.data(),
+                                  w, w, e - s, hipMemcpyHostToDevice) != hipSuccess) {
+                fclose(f);
+                return fail(LQG_ERR_DEVICE, "sidecar upload failed");
+            }
+        }
+    }
+    fclose(f);
+    if (!ok) return fail(LQG_ERR_IO, std::string("short read from ") + path);
+    h->h2_ready = true;
+    return LQG_OK;
+}
+
+lqg_status lqg_h2_probe(lqg_index* h, const float* queries, uint64_t nq, const uint32_t* nodes,
+                        uint8_t* lut, float* lo_width, int32_t* sumq, float* sqr_y, float* est) {
+    if (!h || !queries || !lut || !lo_width || !sumq) return fail(LQG_ERR_ARG, "null argument");
+    if (nodes && (!sqr_y || !est)) return fail(LQG_ERR_ARG, "nodes given without sqr_y / est");
+    if (nq == 0 || nq > 65535) return fail(LQG_ERR_ARG, "nq must be in [1, 65535]");
+    if (!h->h2_ready) return fail(LQG_ERR_STATE, "no sidecar");
+    if (nodes)
+        for (uint64_t i = 0; i < nq; ++i)
+            if (nodes[i] >= h->n) return fail(LQG_ERR_ARG, "node id out of range");
+    const uint32_t p = h->h2_p, deg = static_cast<uint32_t>(h->deg);
+    lqg::DevBuf<float> dq, dpar, dsq, dest;
+    lqg::DevBuf<uint32_t> dn;
+    lqg::DevBuf<uint8_t> dlut;
+    LQG_HIP_TRY(dq.reserve(nq * h->dim));
+    LQG_HIP_TRY(dpar.reserve(3 * nq));
+    LQG_HIP_TRY(dlut.reserve(nq * 4ull * p));
+    LQG_HIP_TRY(hipMemcpy(dq.p, queries, nq * h->dim * 4, hipMemcpyHostToDevice));
+    if (nodes) {
+        LQG_HIP_TRY(dn.reserve(nq));
+        LQG_HIP_TRY(dsq.reserve(nq));
+        LQG_HIP_TRY(dest.reserve(nq * deg));
+        LQG_HIP_TRY(hipMemcpy(dn.p, nodes, nq * 4, hipMemcpyHostToDevice));
+    }
+    LQG_HIP_TRY(lqg::h2_launch_probe(h->h2dev(), dq.p, static_cast<uint32_t>(nq),
+                                     nodes ? dn.p : nullptr, dlut.p, dpar.p, dsq.p, dest.p,
+                                     h->stream));
+    LQG_HIP_TRY(hipStreamSynchronize(h->stream));
+    std::vector<float> par(3 * nq);
+    LQG_HIP_TRY(hipMemcpy(par.data(), dpar.p, 12 * nq, hipMemcpyDeviceToHost));
+    LQG_HIP_TRY(hipMemcpy(lut, dlut.p, nq * 4ull * p, hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < nq; ++i) {
+        lo_width[2 * i] = par[3 * i];
+        lo_width[2 * i + 1] = par[3 * i + 1];
+        memcpy(&sumq[i], &par[3 * i + 2], 4);
+    }
+    if (nodes) {
+        LQG_HIP_TRY(hipMemcpy(sqr_y, dsq.p, 4 * nq, hipMemcpyDeviceToHost));
+        LQG_HIP_TRY(hipMemcpy(est, dest.p, 4 * nq * deg, hipMemcpyDeviceToHost));
+    }
+    return LQG_OK;
+}
+
+}  // extern "C"

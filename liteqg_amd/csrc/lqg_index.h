@@ -85,6 +85,34 @@ struct lqg_index {
     lqg::LaunchCfg launch[5];
     bool launch_set[5] = {false, false, false, false, false};
 
+    // FastScan RaBitQ search (search_qg, qg.hpp:252-287): node records on the
+    // device (raw | codes | factors | ids) and the rotation signs
+    lqg_mode mode = LQG_MODE_SQ8;
+    bool h2_ready = false;
+    uint32_t h2_p = 0, h2_stride = 0, h2_off_codes = 0, h2_off_fac = 0, h2_off_ids = 0;
+    std::vector<float> h2_signs;
+    lqg::DevBuf<uint8_t> d_h2rec;
+    lqg::DevBuf<float> d_h2signs;
+    lqg::DevBuf<uint32_t> d_h2gvis, d_h2err;
+    lqg::H2Cfg h2_launch[5];
+    bool h2_launch_set[5] = {false, false, false, false, false};
+
+    lqg::H2Dev h2dev() const {
+        lqg::H2Dev ix;
+        ix.rec = d_h2rec.p;
+        ix.signs = d_h2signs.p;
+        ix.n = static_cast<uint32_t>(n);
+        ix.dim = static_cast<uint32_t>(dim);
+        ix.deg = static_cast<uint32_t>(deg);
+        ix.p = h2_p;
+        ix.ep = ep;
+        ix.stride = h2_stride;
+        ix.off_codes = h2_off_codes;
+        ix.off_fac = h2_off_fac;
+        ix.off_ids = h2_off_ids;
+        return ix;
+    }
+
     lqg::DevIndex dev() const {
         lqg::DevIndex ix;
         ix.qdata = d_qdata.p;
@@ -110,6 +138,20 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
                      const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, lqg_order order,
                      uint32_t* d_results, float* d_dists, uint32_t* d_stats, hipStream_t s,
                      bool count_visits = false);
+
+// FastScan counterpart of run_batch (lqg_h2.cpp); d_stats gets (E, pops).
+lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
+                        const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, uint32_t* d_results,
+                        float* d_dists, uint32_t* d_stats, hipStream_t s);
+
+// Launch settings of the FastScan kernel used when none were set for a class.
+H2Cfg h2_default_launch(int ef_cap);
+
+// Frees the FastScan records (index reloaded or rebuilt).
+void h2_reset(lqg_index* h);
+
+// After a synchronised FastScan launch: fails if a visited set overflowed.
+lqg_status h2_check_error(lqg_index* h);
 
 // Device graph build for lqg_build_index.
 lqg_status build_index(lqg_index* h, const float* data, uint32_t ef_build, uint32_t num_iter);

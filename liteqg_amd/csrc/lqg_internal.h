@@ -47,8 +47,62 @@ struct LaunchCfg {
     int occupancy;       // 8: kernel variant capped at 64 VGPRs (8 waves/SIMD); 0: uncapped
 };
 
+// Device view of an index for the FastScan RaBitQ search (search_qg,
+// qg.hpp:252-287).  One record per node, `stride` bytes (a multiple of 128):
+//   raw f32[dim] | codes[deg * p/8] | triple_x f32[deg] | fac_dq f32[deg] |
+//   fac_vq f32[deg] | ids u32[deg]
+// so that expanding a node is one contiguous read.  Codes are vector-major:
+// neighbour j's p/8 bytes, byte b holding codebook 2b (dims 8b..8b+3, bit 3 =
+// dim 8b) in its low nibble and codebook 2b+1 in its high nibble.
+struct H2Dev {
+    const uint8_t* rec;
+    const float* signs;     // p rotation signs, +-1/sqrt(p)
+    uint32_t n, dim, deg, p, ep;
+    uint32_t stride, off_codes, off_fac, off_ids;  // bytes
+};
+
+// One batch of queries for the FastScan search kernel.
+struct H2Job {
+    const float* queries;   // nq x dim
+    const uint32_t* ef;     // per query, or null -> ef_all
+    const uint32_t* k;      // per query, or null -> k_all
+    uint32_t nq, ef_all, k_all, kstride;
+    uint32_t* out_ids;      // nq x kstride
+    float* out_d;           // nq x kstride or null
+    uint32_t* out_stats;    // nq x 2 (expansions, pops) or null
+    uint32_t* counter;      // work-queue head, zeroed before launch
+    uint32_t* gvis;         // per-wave overflow visited tables (gvis_bits each)
+    uint32_t gvis_bits;
+    uint32_t* err;          // set to 1 if a visited set overflows both tables
+};
+
+struct H2Cfg {
+    int ef_cap;          // 64 .. 1024
+    int waves_per_group; // 1..4
+    int groups;          // persistent grid size (0 = fill the device)
+    int hash_bits;       // log2 entries of the per-wave LDS visited table
+};
+
 // Returns LDS bytes one workgroup needs for this configuration.
 size_t search_lds_bytes(const DevIndex& ix, const LaunchCfg& cfg);
+
+// FastScan search launches (h2_search.hip).
+size_t h2_lds_bytes(const H2Dev& ix, const H2Cfg& cfg, uint32_t kmax);
+int h2_groups(const H2Dev& ix, const H2Cfg& cfg, uint32_t kmax, uint32_t nq);
+hipError_t h2_launch_search(const H2Dev& ix, const H2Job& job, const H2Cfg& cfg,
+                            hipStream_t stream);
+// Query preparation (and optionally one node's FastScan estimates) for each
+// query: lut (nq x 4p), params (nq x 3: lo, width, sumq bits), sqr_y (nq), est
+// (nq x deg); nodes may be null (no scan).
+hipError_t h2_launch_probe(const H2Dev& ix, const float* queries, uint32_t nq,
+                           const uint32_t* nodes, uint8_t* lut, float* params, float* sqr_y,
+                           float* est, hipStream_t stream);
+// Sidecar build: rotates every raw vector, then fills codes and factors of
+// every record (raw and ids must already be in the records).
+hipError_t h2_launch_sidecar(const H2Dev& ix, float* rot_scratch, hipStream_t stream);
+// Copies raw rows (n x dim f32) and neighbour ids (n x deg) into the records.
+hipError_t h2_launch_pack(const H2Dev& ix, const float* raw, const uint32_t* nbr,
+                          hipStream_t stream);
 
 // Enqueues the search kernel (queries pulled from job.counter).
 hipError_t launch_search(const DevIndex& ix, const SearchJob& job, const LaunchCfg& cfg,

@@ -15,7 +15,13 @@ gfx950 kernels.  Differences, all deliberate:
   farthest entries of the final ef-pool in ascending order, exactly as the
   shipped ``search`` does (qg.hpp:289-296,340-343); ``"nearest"`` returns the
   k nearest of the same pool.
-* Additions: ``set_order``, ``search_batch`` (many queries, one launch).
+* Additions: ``set_order``, ``search_batch`` (many queries, one launch), and
+  the FastScan RaBitQ path (``search_qg``, qg.hpp:252-287, which the reference
+  ships but never calls): ``build_sidecar`` / ``set_sidecar`` /
+  ``load_sidecar`` give it its codes and factors, ``set_mode("fastscan")``
+  routes ``search`` and ``search_batch`` to it.  Its results are the k
+  nearest expanded nodes by exact distance, ascending; slots it cannot fill
+  hold 0, as the reference's zero-initialised ResultBuffer does.
 """
 from __future__ import annotations
 
@@ -23,7 +29,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import ORDER, Launch, LqgError, Stats, check, lib, ptr
+from ._lib import MODE, ORDER, Launch, LqgError, Stats, check, lib, ptr
 
 UINT_MAX = 0xFFFFFFFF
 
@@ -155,6 +161,86 @@ class Index:
         Q = np.ascontiguousarray(queries, dtype=np.float32).reshape(-1, self.dimension)
         out = np.empty((Q.shape[0], 2), dtype=np.uint32)
         check(lib().lqg_count_visits(self._h, ptr(Q), Q.shape[0], int(ef), int(k), ptr(out)))
+        return out
+
+    # -- FastScan RaBitQ path (search_qg) ----------------------------------
+    @property
+    def padded_dim(self) -> int:
+        p = 1
+        while p < self.dimension:
+            p <<= 1
+        return p
+
+    def set_mode(self, mode: str) -> None:
+        """``"sq8"`` (the reference's live search, default) or ``"fastscan"``."""
+        if mode not in MODE:
+            raise ValueError(f"mode must be 'sq8' or 'fastscan', got {mode!r}")
+        check(lib().lqg_set_mode(self._h, MODE[mode]))
+
+    def get_mode(self) -> str:
+        m = C.c_int()
+        check(lib().lqg_get_mode(self._h, C.byref(m)))
+        return {v: k for k, v in MODE.items()}[m.value]
+
+    def build_sidecar(self, signs=None, seed: int = 0) -> None:
+        """RaBitQ codes and factors of every node's neighbours, built on the
+        device; rotation signs given (P floats) or drawn from ``seed``."""
+        s = None
+        if signs is not None:
+            s = np.ascontiguousarray(signs, dtype=np.float32).reshape(-1)
+            if s.size != self.padded_dim:
+                raise ValueError(f"signs must have {self.padded_dim} values")
+        check(lib().lqg_h2_build_sidecar(self._h, ptr(s), int(seed)))
+
+    def set_sidecar(self, signs, codes, factors, layout: str = "vector") -> None:
+        """Install a sidecar: codes ``uint8[N, deg*P/8]`` in the reference's
+        FastScan block layout (``layout="blocks"``) or vector-major
+        (``"vector"``), factors ``float32[N, 3*deg]``."""
+        p, g, n = self.padded_dim, self.degree_bound, self.num_elements
+        s = np.ascontiguousarray(signs, dtype=np.float32).reshape(-1)
+        c = np.ascontiguousarray(codes, dtype=np.uint8).reshape(n, g * p // 8)
+        f = np.ascontiguousarray(factors, dtype=np.float32).reshape(n, 3 * g)
+        if s.size != p:
+            raise ValueError(f"signs must have {p} values")
+        if layout not in ("blocks", "vector"):
+            raise ValueError("layout must be 'blocks' or 'vector'")
+        check(lib().lqg_h2_set_sidecar(self._h, ptr(s), ptr(c), 0 if layout == "blocks" else 1,
+                                       ptr(f)))
+
+    def get_sidecar(self):
+        """(signs, codes vector-major, factors) of the installed sidecar."""
+        p, g, n = self.padded_dim, self.degree_bound, self.num_elements
+        s = np.empty(p, np.float32)
+        c = np.empty((n, g * p // 8), np.uint8)
+        f = np.empty((n, 3 * g), np.float32)
+        check(lib().lqg_h2_get_sidecar(self._h, ptr(s), ptr(c), ptr(f)))
+        return s, c, f
+
+    def save_sidecar(self, path: str) -> None:
+        check(lib().lqg_h2_save_sidecar(self._h, str(path).encode()))
+
+    def load_sidecar(self, path: str) -> None:
+        check(lib().lqg_h2_load_sidecar(self._h, str(path).encode()))
+
+    def fastscan_probe(self, queries, nodes=None):
+        """Device query preparation (and, with nodes, the exact distance to
+        nodes[i] and its neighbours' FastScan estimates) for parity checks.
+        Returns dict(lut, lo, width, sumq[, sqr_y, est])."""
+        Q = np.ascontiguousarray(queries, dtype=np.float32).reshape(-1, self.dimension)
+        nq, p = Q.shape[0], self.padded_dim
+        lut = np.empty((nq, 4 * p), np.uint8)
+        lw = np.empty((nq, 2), np.float32)
+        sq = np.empty(nq, np.int32)
+        nd = sy = est = None
+        if nodes is not None:
+            nd = np.ascontiguousarray(nodes, dtype=np.uint32).reshape(nq)
+            sy = np.empty(nq, np.float32)
+            est = np.empty((nq, self.degree_bound), np.float32)
+        check(lib().lqg_h2_probe(self._h, ptr(Q), nq, ptr(nd), ptr(lut), ptr(lw), ptr(sq),
+                                 ptr(sy), ptr(est)))
+        out = dict(lut=lut, lo=lw[:, 0].copy(), width=lw[:, 1].copy(), sumq=sq)
+        if nodes is not None:
+            out.update(sqr_y=sy, est=est)
         return out
 
     def info(self) -> dict:

@@ -1,0 +1,233 @@
+"""Device FastScan RaBitQ path (search_qg, qg.hpp:252-287) against the
+reference's outputs and the oracle.
+
+Bar: query preparation (LUT bytes, lo, width, sumq), exact distances and
+FastScan estimates bit-identical to the reference's on the h2_* fixtures
+(which hold the reference's own sidecar); result ids identical to the
+reference's search_qg on every fixture query and ef; on random indexes, ids,
+distances and (expansions, pops) identical to the oracle's orc_search_h2 over
+the same sidecar; the device-built sidecar equal to the oracle's (codes bit
+for bit, factors within FACTOR_TOL of their magnitude).
+"""
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, TIE_FREE, load_golden, make_random_index
+
+pytestmark = pytest.mark.gpu
+
+H2_NAMES = TIE_FREE
+# Factors are FP64 sums rounded to float; the device and the oracle sum in the
+# same order, so they are expected to agree exactly.  Allowed: 2e-6 of the
+# largest magnitude of each factor array (the oracle-vs-reference bound).
+FACTOR_TOL = 2e-6
+
+
+def _index_from_golden(name):
+    from liteqg_amd.symphonyqg import Index
+
+    z = np.load(f"{GOLDEN}/{name}.npz")
+    n, d, deg = (int(x) for x in z["shape"])
+    ix = Index("QG", "L2", n, d, deg)
+    ix.load(f"{GOLDEN}/{name}.index")
+    return ix
+
+
+def _index_from_arrays(arr, tmp_path, tag):
+    from liteqg_amd.symphonyqg import Index
+    from oracle import oracle as O
+
+    p = tmp_path / f"{tag}.index"
+    O.write_index(str(p), arr)
+    ix = Index("QG", "L2", arr.n, arr.dim, arr.deg)
+    ix.load(str(p))
+    return ix
+
+
+def _signs(p, seed):
+    rng = np.random.default_rng(seed)
+    return ((rng.integers(0, 2, p) * 2 - 1) / np.sqrt(np.float32(p))).astype(np.float32)
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.mark.parametrize("name", H2_NAMES)
+def test_probe_matches_reference(name):
+    """LUT, lo, width, sumq, sqr_y and every estimate of the entry point's row,
+    computed on the device from the reference's sidecar, equal the
+    reference's (qg_query.hpp:28-43, qg_scanner.hpp:99-155)."""
+    arr, _ = load_golden(name)
+    h = np.load(f"{GOLDEN}/h2_{name}.npz")
+    ix = _index_from_golden(name)
+    ix.set_sidecar(h["signs"], h["codes"], h["factors"], layout="blocks")
+    nprep = h["prep_lut"].shape[0]
+    out = ix.fastscan_probe(h["queries"][:nprep], nodes=np.full(nprep, arr.ep, np.uint32))
+    assert np.array_equal(out["lut"], h["prep_lut"])
+    assert np.array_equal(_bits(out["lo"]), _bits(h["prep_lo"]))
+    assert np.array_equal(_bits(out["width"]), _bits(h["prep_width"]))
+    assert np.array_equal(out["sumq"], h["prep_sumq"])
+    assert np.array_equal(_bits(out["sqr_y"]), _bits(h["prep_sqr_y"]))
+    assert np.array_equal(_bits(out["est"]), _bits(h["prep_est"]))
+
+
+@pytest.mark.parametrize("name", H2_NAMES)
+def test_search_matches_reference(name):
+    """search_qg ids for every fixture query at every ef, reference sidecar."""
+    h = np.load(f"{GOLDEN}/h2_{name}.npz")
+    ix = _index_from_golden(name)
+    ix.set_sidecar(h["signs"], h["codes"], h["factors"], layout="blocks")
+    ix.set_mode("fastscan")
+    k = int(h["k"])
+    for ef in h["efs"]:
+        ids = ix.search_batch(h["queries"], k, ef=int(ef))
+        assert np.array_equal(ids, h[f"ids_ef{ef}"]), f"ef={ef}"
+    # the plain per-query entry point runs the same path
+    ix.set_ef(int(h["efs"][1]))
+    for i in range(5):
+        assert np.array_equal(ix.search(h["queries"][i], k), h[f"ids_ef{h['efs'][1]}"][i])
+
+
+@pytest.mark.parametrize("name", H2_NAMES)
+def test_device_sidecar_matches_oracle_and_reference(oracle_mod, name):
+    """The device-built sidecar equals the oracle's; searching with it gives
+    the reference's ids on the fixtures."""
+    O = oracle_mod
+    arr, _ = load_golden(name)
+    h = np.load(f"{GOLDEN}/h2_{name}.npz")
+    ix = _index_from_golden(name)
+    ix.build_sidecar(signs=h["signs"])
+    s, codes, fac = ix.get_sidecar()
+    assert np.array_equal(s, h["signs"])
+    oc, of = O.sidecar_build(arr, h["signs"])
+    assert np.array_equal(codes, oc)
+    g = arr.deg
+    a, b = fac.reshape(-1, 3, g), of.reshape(-1, 3, g)
+    for j in range(3):
+        assert np.abs(a[:, j] - b[:, j]).max() <= FACTOR_TOL * np.abs(b[:, j]).max()
+    ix.set_mode("fastscan")
+    for ef in h["efs"]:
+        assert np.array_equal(ix.search_batch(h["queries"], int(h["k"]), ef=int(ef)),
+                              h[f"ids_ef{ef}"]), f"ef={ef}"
+
+
+def _oracle_h2(O, arr, ix, Q, ef, k):
+    s, codes, fac = ix.get_sidecar()
+    H = O.H2Arrays(arr, s, codes, fac, 1)
+    return O.search_h2_batch(H, Q, ef, k)
+
+
+# (n, d, deg, kind, knn, dup_rate): covers code words per lane 1, 2, 4, 8, 16
+# and 32, a dimension that is not a multiple of 16 (tail FMA), duplicate ids
+# inside neighbour rows, and random as well as kNN graphs
+SHAPES = [
+    (1500, 64, 32, "uniform", True, 0.0),
+    (1500, 100, 32, "uniform", True, 0.05),
+    (1200, 128, 32, "int", True, 0.0),
+    (1000, 96, 64, "uniform", False, 0.1),
+    (800, 200, 32, "uniform", True, 0.0),
+    (800, 256, 64, "uniform", True, 0.0),
+    (500, 768, 32, "uniform", True, 0.0),
+    (400, 960, 64, "uniform", True, 0.0),
+]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: f"n{s[0]}_d{s[1]}_deg{s[2]}")
+def test_search_matches_oracle(oracle_mod, tmp_path, shape):
+    O = oracle_mod
+    n, d, deg, kind, knn, dup = shape
+    arr, _ = make_random_index(n, d, deg, seed=d + deg, kind=kind, knn=knn, dup_rate=dup)
+    ix = _index_from_arrays(arr, tmp_path, "r")
+    p = ix.padded_dim
+    ix.build_sidecar(signs=_signs(p, d))
+    _, codes, fac = ix.get_sidecar()
+    oc, of = O.sidecar_build(arr, _signs(p, d))
+    assert np.array_equal(codes, oc)
+    ix.set_mode("fastscan")
+    rng = np.random.default_rng(d)
+    Q = (arr.raw[rng.integers(0, n, 40)] + rng.normal(0, 0.05, (40, d))).astype(np.float32)
+    for ef, k in ((16, 10), (64, 10), (200, 20), (700, 10)):
+        ids, dis, st = ix.search_batch(Q, k, ef=ef, return_distances=True, return_stats=True)
+        e_ids, e_dis, e_st = _oracle_h2(O, arr, ix, Q, ef, k)
+        assert np.array_equal(ids, e_ids), f"ef={ef}"
+        assert np.array_equal(_bits(dis), _bits(e_dis)), f"ef={ef}"
+        assert st["expansions"] == int(e_st[:, 0].sum())
+        assert st["evaluations"] == int(e_st[:, 1].sum())
+
+
+def test_update_results_and_uneven_ef_k(oracle_mod, tmp_path):
+    """ef below k runs update_results (qg.hpp:395-418), including rows of node
+    0 from the zero-filled id snapshot; per-query ef and k in one launch."""
+    O = oracle_mod
+    arr, _ = make_random_index(1200, 64, 32, seed=11, knn=True, dup_rate=0.05)
+    ix = _index_from_arrays(arr, tmp_path, "u")
+    ix.build_sidecar(seed=5)
+    ix.set_mode("fastscan")
+    rng = np.random.default_rng(3)
+    Q = rng.random((60, 64), dtype=np.float32)
+    ef = rng.integers(1, 300, 60).astype(np.uint32)
+    ef[:10] = 1
+    k = rng.integers(1, 40, 60).astype(np.uint32)
+    k[:10] = 30
+    ids, dis = ix.search_batch(Q, k, ef=ef, return_distances=True)
+    kmax = int(k.max())
+    for i in range(60):
+        e_ids, e_dis, _ = _oracle_h2(O, arr, ix, Q[i:i + 1], int(ef[i]), int(k[i]))
+        assert np.array_equal(ids[i, :k[i]], e_ids[0]), i
+        assert np.array_equal(_bits(dis[i, :k[i]]), _bits(e_dis[0])), i
+        assert np.all(ids[i, k[i]:kmax] == 0xFFFFFFFF)
+
+
+def test_visited_overflow_table(oracle_mod, tmp_path):
+    """A 64-entry LDS visited table moves to device memory after 32
+    expansions; results do not change."""
+    O = oracle_mod
+    arr, _ = make_random_index(1500, 64, 32, seed=2, knn=True)
+    ix = _index_from_arrays(arr, tmp_path, "o")
+    ix.build_sidecar(seed=9)
+    ix.set_mode("fastscan")
+    rng = np.random.default_rng(4)
+    Q = rng.random((30, 64), dtype=np.float32)
+    ix.set_launch(0, waves_per_group=2, cache_bits=6)
+    ids, dis, st = ix.search_batch(Q, 10, ef=256, return_distances=True, return_stats=True)
+    e_ids, e_dis, e_st = _oracle_h2(O, arr, ix, Q, 256, 10)
+    assert int(e_st[:, 0].max()) > 32  # the overflow path ran
+    assert np.array_equal(ids, e_ids)
+    assert np.array_equal(_bits(dis), _bits(e_dis))
+    ix.reset_launch(0)
+
+
+def test_sidecar_file_roundtrip(tmp_path):
+    from liteqg_amd._lib import LqgError
+    from liteqg_amd.symphonyqg import Index
+
+    arr, _ = make_random_index(900, 64, 32, seed=7, knn=True)
+    ix = _index_from_arrays(arr, tmp_path, "s")
+    ix.build_sidecar(seed=1)
+    ix.set_mode("fastscan")
+    Q = np.random.default_rng(0).random((20, 64), dtype=np.float32)
+    want = ix.search_batch(Q, 10, ef=64)
+    ix.save_sidecar(str(tmp_path / "s.side"))
+    ix2 = Index("QG", "L2", arr.n, arr.dim, arr.deg)
+    ix2.load(str(tmp_path / "s.index"))
+    ix2.load_sidecar(str(tmp_path / "s.side"))
+    ix2.set_mode("fastscan")
+    assert np.array_equal(ix2.search_batch(Q, 10, ef=64), want)
+    other, _ = make_random_index(900, 64, 32, seed=8, knn=True)
+    ix3 = _index_from_arrays(other, tmp_path, "t")
+    with pytest.raises(LqgError):
+        ix3.load_sidecar(str(tmp_path / "s.side"))
+
+
+def test_mode_switch_keeps_sq8_results(tmp_path):
+    arr, _ = make_random_index(1000, 64, 32, seed=3, knn=True)
+    ix = _index_from_arrays(arr, tmp_path, "m")
+    Q = np.random.default_rng(1).random((30, 64), dtype=np.float32)
+    before = ix.search_batch(Q, 10, ef=50, order="nearest")
+    ix.build_sidecar(seed=2)
+    ix.set_mode("fastscan")
+    assert ix.get_mode() == "fastscan"
+    ix.search_batch(Q, 10, ef=50)
+    ix.set_mode("sq8")
+    assert np.array_equal(ix.search_batch(Q, 10, ef=50, order="nearest"), before)
