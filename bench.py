@@ -33,6 +33,9 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 EF_SWEEP = (20, 50, 100, 200, 500)
+FS_EF_SWEEP = (50, 100, 200, 300, 500, 700, 1000)
+SERVE_RATES = (100_000.0, 1_000_000.0, 4_000_000.0)
+SERVE_LANES, SERVE_BATCH, SERVE_SLOTS = 4, 2048, 16384
 TARGET_RECALL = 0.95
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "h1_search_kernel_c2_traffic.md")
 
@@ -120,6 +123,107 @@ def time_launches(db, ef, steps, warmup):
     return t0.elapsed_time(t1) / steps, wall
 
 
+def graph_leg(ix, db, ef, args):
+    """K replays of the batch launch captured into a HIP graph (HIP events on
+    the replay stream)."""
+    import torch
+
+    from liteqg_amd.serving import BatchGraph
+
+    g = BatchGraph(ix, db.q.data_ptr(), db.nq, db.ids.data_ptr(), db.k, ef)
+    for _ in range(args.warmup):
+        g.launch(db.stream.cuda_stream)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(db.stream)
+    for _ in range(args.steps):
+        g.launch(db.stream.cuda_stream)
+    t1.record(db.stream)
+    torch.cuda.synchronize()
+    g.close()
+    return t0.elapsed_time(t1) / args.steps
+
+
+def serving_leg(ix, Q, ef, k, args):
+    """The serving engine under Poisson arrivals of single-query requests at
+    fixed rates, plus one unthrottled run (every request at t = 0).  Latency
+    is completion minus scheduled arrival; throughput is requests over the
+    span from the first arrival to the last completion."""
+    from liteqg_amd.serving import Server, poisson_arrivals
+
+    out = {"requests_per_rate": int(Q.shape[0]), "ef": ef, "k": k, "rates": []}
+    with Server(ix, lanes=SERVE_LANES, max_batch=SERVE_BATCH, slots=SERVE_SLOTS, ef_max=ef,
+                kmax=k) as srv:
+        srv.replay(Q[:2000], ef, k, np.zeros(2000, np.uint64))  # warm-up
+        for rate in SERVE_RATES + (None,):
+            arr = (np.zeros(Q.shape[0], np.uint64) if rate is None
+                   else poisson_arrivals(Q.shape[0], rate, seed=args.seed + 2))
+            lat, _ = srv.replay(Q, ef, k, arr)
+            span = float((arr.astype(np.float64) + lat).max() - float(arr[0])) / 1e9
+            row = {"rate_per_s": rate if rate is not None else "unthrottled",
+                   "p50_ms": round(float(np.percentile(lat, 50)) / 1e6, 4),
+                   "p99_ms": round(float(np.percentile(lat, 99)) / 1e6, 4),
+                   "achieved_qps": round(Q.shape[0] / span, 1)}
+            out["rates"].append(row)
+            log(f"serving {row}")
+        out["engine"] = dict(srv.stats(), lanes=SERVE_LANES, max_batch=SERVE_BATCH,
+                             slots=SERVE_SLOTS)
+    return out
+
+
+def fastscan_leg(ix, db, Q, gt, k, args):
+    """The FastScan RaBitQ path (search_qg) on the same index and queries:
+    device sidecar build, ef sweep to recall@10 >= 0.95, batched timing, and
+    its roofline on algorithmic bytes per query
+      4D (query) + E * (4D + deg*P/8 + 16*deg) (node records) + 4k (ids out)
+    with E the exact expansion count the kernel reports."""
+    import torch
+
+    d, deg = ix.dimension, ix.degree_bound
+    p = ix.padded_dim
+    t = time.perf_counter()
+    ix.build_sidecar(seed=args.seed)
+    torch.cuda.synchronize()
+    side_s = time.perf_counter() - t
+    ix.set_mode("fastscan")
+    sweep, chosen = [], None
+    for ef in FS_EF_SWEEP:
+        db.launch(ef)
+        torch.cuda.synchronize()
+        rec = recall_at_k(db.results(), gt, k)
+        ms, _ = time_launches(db, ef, 2, 0)
+        sweep.append(dict(ef=ef, recall=round(rec, 4), qps=round(args.nq / ms * 1e3, 1)))
+        log(f"fastscan ef={ef} recall={rec:.4f} qps={args.nq / ms * 1e3:.0f}")
+        if rec >= TARGET_RECALL:
+            chosen = ef
+            break
+    if chosen is None:
+        chosen = sweep[-1]["ef"]
+    rec = next(s["recall"] for s in sweep if s["ef"] == chosen)
+    tuned = ix.autotune(Q, chosen, k)
+    ms_step, _ = time_launches(db, chosen, args.steps, args.warmup)
+    qps = args.nq / (ms_step / 1e3)
+    db.launch(chosen, with_stats=True)
+    torch.cuda.synchronize()
+    E, pops = db.counters()
+    rec_bytes = 4 * d + deg * p // 8 + 16 * deg
+    alg = float(np.sum(4 * d + E * rec_bytes + 4 * k))
+    achieved = alg / (ms_step / 1e3) / 1e9
+    ix.set_mode("sq8")
+    log(f"fastscan timed ef={chosen}: {ms_step:.3f} ms/step, {qps:.0f} QPS, "
+        f"{achieved:.0f} GB/s")
+    return {"path": "FastScan RaBitQ search_qg (1-bit codes, exact re-rank)", "ef": chosen,
+            "recall_at_10": rec, "qps": round(qps, 1), "ms_per_step": round(ms_step, 4),
+            "ef_sweep": sweep, "sidecar_build_s": round(side_s, 2), "launch": tuned,
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "bytes_per_query": round(alg / args.nq, 1),
+                         "record_bytes": rec_bytes,
+                         "expansions_per_query": round(float(E.mean()), 2),
+                         "pops_per_query": round(float(pops.mean()), 2)}}
+
+
 def read_traffic():
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
     if not os.path.exists(TRAFFIC_FILE):
@@ -175,6 +279,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--ef", type=int, default=0, help="skip the sweep and use this ef")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fastscan", action="store_true", help="skip the FastScan RaBitQ leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--plain", type=int, default=500, help="queries timed on the plain path")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: build, 3 launches at --ef (default 50) with "
@@ -244,6 +350,10 @@ def main():
     E, V = db.counters()
     log(f"timed ef={chosen}: {ms_step:.3f} ms/step, {qps:.0f} QPS")
 
+    # the same launch captured into a HIP graph and replayed
+    graph_ms = graph_leg(ix, db, chosen, args)
+    log(f"graph replay: {graph_ms:.3f} ms/step")
+
     # plain path: one query per Index.search call
     ix.set_ef(chosen)
     ix.set_order("nearest")
@@ -268,6 +378,10 @@ def main():
     alg_bytes = float(np.sum(4 * d + 8 * d + 4 * deg * E + d * V_ref + 4 * k))
     achieved = alg_bytes / (ms_step / 1e3) / 1e9
     traffic = read_traffic()
+
+    serving = None if args.no_serve else serving_leg(ix, Q, chosen, k, args)
+
+    fastscan = None if args.no_fastscan else fastscan_leg(ix, db, Q, gt, k, args)
 
     cpu = None
     if not args.no_cpu:
@@ -310,6 +424,12 @@ def main():
                      "kernel_evaluations_per_query": round(float(V.mean()), 2)},
         "cpu_baseline": cpu,
     }
+    line["graph_replay"] = {"ms_per_step": round(graph_ms, 4),
+                            "qps": round(args.nq / (graph_ms / 1e3), 1)}
+    if serving:
+        line["serving"] = serving
+    if fastscan:
+        line["fastscan"] = fastscan
     if cpu:
         line["gpu_over_cpu"] = round(qps / cpu["value"], 1)
     print(json.dumps(line), flush=True)

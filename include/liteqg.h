@@ -27,7 +27,8 @@ typedef enum {
     LQG_ERR_IO = 3,      /* file missing, short, or of the wrong size          */
     LQG_ERR_DEVICE = 4,  /* HIP runtime error or no gfx950 device              */
     LQG_ERR_NOMEM = 5,   /* host or device allocation failed                   */
-    LQG_ERR_UNSUPPORTED = 6
+    LQG_ERR_UNSUPPORTED = 6,
+    LQG_ERR_BUSY = 7     /* serving: no free request slot, or result not ready  */
 } lqg_status;
 
 /* Output order of the k ids (see DESIGN: the reference returns the k farthest
@@ -114,11 +115,81 @@ lqg_status lqg_search_batch(lqg_index* index, const float* queries, uint64_t nq,
  * `stream` (a hipStream_t, NULL = the handle's stream) and not synchronised.
  * d_stats (optional) receives 2 x nq uint32 counters per query: (E, V) in SQ8
  * mode, (E, pops) in FastScan mode (E = 0xFFFFFFFF marks a query whose
- * visited set overflowed; the host entry points fail instead). */
+ * visited set overflowed; the host entry points fail instead).
+ * Every call on one index uses the index's single work-queue counter: two
+ * calls must not run at the same time (for concurrent launches use graphs,
+ * lqg_graph_create, or the serving engine, lqg_serve_start, which own their
+ * scratch). */
 lqg_status lqg_search_batch_device(lqg_index* index, const float* d_queries, uint64_t nq,
                                    const uint32_t* d_ef, const uint32_t* d_k, uint32_t ef_max,
                                    uint32_t kmax, lqg_order order, uint32_t* d_results,
                                    float* d_dists, uint32_t* d_stats, void* stream);
+
+/* A batch launch captured into a HIP graph: lqg_search_batch_device with
+ * fixed arguments (device arrays whose contents may change between replays),
+ * recorded once and replayed with one hipGraphLaunch.  The graph holds the
+ * work-queue reset and the persistent search kernel (kernel nodes only), in
+ * the mode and with the launch settings in force at creation.  It refers to
+ * the index's device memory: destroy it before the index is reloaded, rebuilt
+ * or destroyed.  No counterpart in the reference. */
+typedef struct lqg_batch_graph lqg_batch_graph;
+
+lqg_status lqg_graph_create(lqg_index* index, const float* d_queries, uint64_t nq,
+                            const uint32_t* d_ef, const uint32_t* d_k, uint32_t ef_max,
+                            uint32_t kmax, lqg_order order, uint32_t* d_results, float* d_dists,
+                            uint32_t* d_stats, lqg_batch_graph** out);
+/* Enqueues one replay on `stream` (a hipStream_t, NULL = the index's stream). */
+lqg_status lqg_graph_launch(lqg_batch_graph* graph, void* stream);
+lqg_status lqg_graph_destroy(lqg_batch_graph* graph);
+
+/* Serving engine: requests (one query each, its own ef and k) are submitted
+ * as they arrive and complete asynchronously.  A dispatcher thread keeps up to
+ * `lanes` batch launches in flight on separate streams; requests that arrive
+ * while launches run are admitted to the next free lane (continuous
+ * batching), so their work overlaps the work already on the device.
+ * Admission: at most `slots` requests outstanding (host pinned request
+ * table); ef <= ef_max and k <= kmax, which fix the kernel's LDS layout for
+ * the life of the server.  The search path is the index's mode at dispatch
+ * time.  Results of a request do not depend on what else is in flight.  No
+ * counterpart in the reference (one query per call, qg.hpp:235-243). */
+typedef struct lqg_server lqg_server;
+
+typedef struct {
+    uint32_t lanes;     /* launches in flight, 1..16                              */
+    uint32_t max_batch; /* requests per launch, 1..65536                          */
+    uint32_t slots;     /* outstanding requests, >= max_batch                     */
+    uint32_t ef_max;    /* largest ef a request may ask, <= 1024                  */
+    uint32_t kmax;      /* largest k a request may ask                            */
+    lqg_order order;    /* SQ8 mode output order                                  */
+} lqg_serve_config;
+
+typedef struct {
+    uint64_t submitted, completed, batches;
+    uint32_t max_lanes_busy; /* most launches seen in flight at once            */
+    uint32_t max_batch_seen; /* largest batch dispatched                       */
+} lqg_serve_stats;
+
+lqg_status lqg_serve_start(lqg_index* index, const lqg_serve_config* cfg, lqg_server** out);
+/* Copies the query (D floats); LQG_ERR_BUSY when every slot is taken. */
+lqg_status lqg_serve_submit(lqg_server* server, const float* query, uint32_t ef, uint32_t k,
+                            uint64_t* ticket);
+/* LQG_OK and the k ids (dists optional) once done, LQG_ERR_BUSY before; the
+ * ticket is released by a successful poll. */
+lqg_status lqg_serve_poll(lqg_server* server, uint64_t ticket, uint32_t* results, float* dists);
+/* Blocks until done, then as lqg_serve_poll. */
+lqg_status lqg_serve_wait(lqg_server* server, uint64_t ticket, uint32_t* results, float* dists);
+lqg_status lqg_serve_get_stats(lqg_server* server, lqg_serve_stats* stats);
+/* Drains in-flight work and frees the server. */
+lqg_status lqg_serve_stop(lqg_server* server);
+
+/* Load replay for latency measurement: submits query i at arrival_ns[i]
+ * (nanoseconds after the call starts; non-decreasing) with ef[i], k[i],
+ * retrying while the server is full, and waits for all.  Writes latency_ns[i]
+ * (completion minus scheduled arrival), results (nq x kmax, 0xFFFFFFFF past
+ * k[i]) if given. */
+lqg_status lqg_serve_replay(lqg_server* server, const float* queries, uint64_t nq,
+                            const uint32_t* ef, const uint32_t* k, const uint64_t* arrival_ns,
+                            uint64_t* latency_ns, uint32_t* results);
 
 /* Launch settings of the search kernel for one pool-capacity class
  * (ef <= 64, 128, 256, 512, 1024).  No counterpart in the reference (CPU

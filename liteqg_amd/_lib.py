@@ -34,6 +34,18 @@ class Stats(C.Structure):
                 ("evaluations", C.c_uint64), ("device_ms", C.c_double)]
 
 
+class ServeConfig(C.Structure):
+    _fields_ = [("lanes", C.c_uint32), ("max_batch", C.c_uint32), ("slots", C.c_uint32),
+                ("ef_max", C.c_uint32), ("kmax", C.c_uint32), ("order", C.c_int)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("batches", C.c_uint64),
+                ("max_lanes_busy", C.c_uint32), ("max_batch_seen", C.c_uint32)]
+
+
+LQG_ERR_BUSY = 7
+
 _lib = None
 
 # name -> (restype, argtypes); every function returns lqg_status (int)
@@ -64,6 +76,17 @@ _SIGS = {
     "lqg_h2_save_sidecar": [_P, C.c_char_p],
     "lqg_h2_load_sidecar": [_P, C.c_char_p],
     "lqg_h2_probe": [_P, _P, C.c_uint64, _P, _P, _P, _P, _P, _P],
+    "lqg_graph_create": [_P, _P, C.c_uint64, _P, _P, C.c_uint32, C.c_uint32, C.c_int, _P, _P, _P,
+                         C.POINTER(_P)],
+    "lqg_graph_launch": [_P, _P],
+    "lqg_graph_destroy": [_P],
+    "lqg_serve_start": [_P, _P, C.POINTER(_P)],
+    "lqg_serve_submit": [_P, _P, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)],
+    "lqg_serve_poll": [_P, C.c_uint64, _P, _P],
+    "lqg_serve_wait": [_P, C.c_uint64, _P, _P],
+    "lqg_serve_get_stats": [_P, _P],
+    "lqg_serve_stop": [_P],
+    "lqg_serve_replay": [_P, _P, C.c_uint64, _P, _P, _P, _P, _P],
 }
 EXPORTED = tuple(_SIGS) + ("lqg_last_error",)
 
@@ -74,6 +97,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -m liteqg_amd.build`")
+        # One HIP runtime serves the whole process: the first libamdhip64 loaded.
+        # PyTorch-ROCm ships its own and cannot initialise its devices on another
+        # one loaded before it, while this library runs on either.  So when torch
+        # is installed it is loaded first, and device tensors can be shared.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, args in _SIGS.items():
             fn = getattr(L, name)

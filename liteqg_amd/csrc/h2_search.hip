@@ -771,17 +771,21 @@ __global__ void __launch_bounds__(256) h2_sidecar_kernel(H2Dev ix, const float* 
                                static_cast<double>(2 * static_cast<int>(pop) - static_cast<int>(ix.p)));
 }
 
-// Raw rows and neighbour ids into the records (one wave per node).
-__global__ void __launch_bounds__(256) h2_pack_kernel(H2Dev ix, const float* raw,
-                                                       const uint32_t* nbr) {
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    const uint32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    if (u >= ix.n) return;
-    unsigned char* rec = const_cast<unsigned char*>(ix.rec) + static_cast<size_t>(u) * ix.stride;
-    float* r = reinterpret_cast<float*>(rec);
-    for (uint32_t i = lane; i < ix.dim; i += kWave) r[i] = raw[static_cast<size_t>(u) * ix.dim + i];
-    uint32_t* ids = reinterpret_cast<uint32_t*>(rec + ix.off_ids);
-    for (uint32_t i = lane; i < ix.deg; i += kWave) ids[i] = nbr[static_cast<size_t>(u) * ix.deg + i];
+// Copies w bytes per node (w a multiple of 4) between a contiguous n x w
+// array and offset `off` of every record; to_rec selects the direction.
+__global__ void __launch_bounds__(256) h2_rows_kernel(unsigned char* rec, uint32_t stride,
+                                                       uint32_t off, unsigned char* buf,
+                                                       uint32_t w, uint32_t n, int to_rec) {
+    const uint32_t words = w / 4;
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<uint64_t>(n) * words) return;
+    const uint64_t u = t / words, i = t % words;
+    uint32_t* r = reinterpret_cast<uint32_t*>(rec + u * stride + off) + i;
+    uint32_t* b = reinterpret_cast<uint32_t*>(buf + u * w) + i;
+    if (to_rec)
+        *r = *b;
+    else
+        *b = *r;
 }
 
 // ---------------------------------------------------------------------------
@@ -906,11 +910,13 @@ hipError_t h2_launch_sidecar(const H2Dev& ix, float* rot, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t h2_launch_pack(const H2Dev& ix, const float* raw, const uint32_t* nbr,
+hipError_t h2_launch_rows(const H2Dev& ix, uint32_t off, void* buf, uint32_t w, bool to_rec,
                           hipStream_t stream) {
-    const uint32_t waves = 4;
-    hipLaunchKernelGGL(h2_pack_kernel, dim3((ix.n + waves - 1) / waves), dim3(kWave * waves), 0,
-                       stream, ix, raw, nbr);
+    if (w % 4 != 0 || off % 4 != 0) return hipErrorInvalidValue;
+    const uint64_t threads = static_cast<uint64_t>(ix.n) * (w / 4);
+    hipLaunchKernelGGL(h2_rows_kernel, dim3(static_cast<uint32_t>((threads + 255) / 256)),
+                       dim3(256), 0, stream, const_cast<unsigned char*>(ix.rec), ix.stride, off,
+                       static_cast<unsigned char*>(buf), w, ix.n, to_rec ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -100,7 +100,7 @@ lqg_status ensure_device(lqg_index* h) {
     LQG_HIP_TRY(hipMemcpy(h->d_nbr.p, dev_nbr.data(), ng * 4, hipMemcpyHostToDevice));
     LQG_HIP_TRY(hipMemcpy(h->d_min.p, mn.data(), h->dim * 4, hipMemcpyHostToDevice));
     LQG_HIP_TRY(hipMemcpy(h->d_scale.p, sc.data(), h->dim * 4, hipMemcpyHostToDevice));
-    LQG_HIP_TRY(h->d_counter.reserve(1));
+    LQG_HIP_TRY(h->scratch.counter.reserve(1));
     h->on_device = true;
     return LQG_OK;
 }
@@ -128,12 +128,14 @@ LaunchCfg default_launch(int ef_cap) {
 lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
                      const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, lqg_order order,
                      uint32_t* d_results, float* d_dists, uint32_t* d_stats, hipStream_t s,
-                     bool count_visits) {
+                     bool count_visits, Scratch* sc) {
+    if (!sc) sc = &h->scratch;
     if (h->mode == LQG_MODE_FASTSCAN) {
         if (count_visits) return fail(LQG_ERR_UNSUPPORTED, "visit counting is for the SQ8 path");
         return run_h2_batch(h, d_queries, nq, d_ef, d_k, ef_max, kmax, d_results, d_dists, d_stats,
-                            s);
+                            s, sc);
     }
+    LQG_HIP_TRY(sc->counter.reserve(1));
     if (ef_class(ef_max) == 0)
         return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported on the device");
     SearchJob job;
@@ -148,7 +150,7 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
     job.out_ids = d_results;
     job.out_d = d_dists;
     job.out_stats = d_stats;
-    job.counter = h->d_counter.p;
+    job.counter = sc->counter.p;
     job.visit_bits = nullptr;
     job.visit_words = 0;
     const int cls = ef_class(ef_max);
@@ -163,7 +165,7 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
         LQG_HIP_TRY(h->d_visit.reserve(words));
         job.visit_bits = h->d_visit.p;
     }
-    LQG_HIP_TRY(launch_reset_counter(h->d_counter.p, s));
+    LQG_HIP_TRY(launch_reset_counter(sc->counter.p, s));
     LQG_HIP_TRY(launch_search(h->dev(), job, cfg, s));
     return LQG_OK;
 }
@@ -226,13 +228,11 @@ lqg_status lqg_destroy(lqg_index* h) {
     h->d_ef.release();
     h->d_k.release();
     h->d_stats.release();
-    h->d_counter.release();
+    h->scratch.release();
     h->d_dist.release();
     h->d_visit.release();
     h->d_h2rec.release();
     h->d_h2signs.release();
-    h->d_h2gvis.release();
-    h->d_h2err.release();
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);

@@ -50,6 +50,29 @@ lqg_status set_layout(lqg_index* h) {
     return LQG_OK;
 }
 
+// Moves w bytes per node between host memory (n x w, contiguous) and offset
+// `off` of every record: one contiguous copy to or from a device staging
+// array, then the row kernel (a pitched host copy of millions of short rows
+// is far slower).
+lqg_status host_rows(lqg_index* h, uint32_t off, void* host, size_t w, bool to_rec) {
+    DevBuf<unsigned char> tmp;
+    LQG_HIP_TRY(tmp.reserve(h->n * w));
+    hipError_t e = hipSuccess;
+    if (to_rec) {
+        e = hipMemcpyAsync(tmp.p, host, h->n * w, hipMemcpyHostToDevice, h->stream);
+        if (e == hipSuccess)
+            e = h2_launch_rows(h->h2dev(), off, tmp.p, static_cast<uint32_t>(w), true, h->stream);
+    } else {
+        e = h2_launch_rows(h->h2dev(), off, tmp.p, static_cast<uint32_t>(w), false, h->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(host, tmp.p, h->n * w, hipMemcpyDeviceToHost, h->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    tmp.release();
+    if (e != hipSuccess) return hip_fail(e, "record row copy");
+    return LQG_OK;
+}
+
 // Records with the raw rows and neighbour ids of the host copy in place
 // (codes and factors still to be written).
 lqg_status alloc_records(lqg_index* h) {
@@ -58,13 +81,12 @@ lqg_status alloc_records(lqg_index* h) {
     h->h2_ready = false;
     LQG_HIP_TRY(h->d_h2rec.reserve(h->n * static_cast<size_t>(h->h2_stride)));
     LQG_HIP_TRY(h->d_h2signs.reserve(h->h2_p));
-    LQG_HIP_TRY(hipMemcpy2DAsync(h->d_h2rec.p, h->h2_stride, h->raw.data(), 4 * h->dim, 4 * h->dim,
-                                 h->n, hipMemcpyHostToDevice, h->stream));
-    LQG_HIP_TRY(hipMemcpy2DAsync(h->d_h2rec.p + h->h2_off_ids, h->h2_stride, h->nbr.data(),
-                                 4 * h->deg, 4 * h->deg, h->n, hipMemcpyHostToDevice, h->stream));
-    LQG_HIP_TRY(hipMemcpyAsync(h->d_h2signs.p, h->h2_signs.data(), 4ull * h->h2_p,
-                               hipMemcpyHostToDevice, h->stream));
-    LQG_HIP_TRY(hipStreamSynchronize(h->stream));
+    st = host_rows(h, 0, h->raw.data(), 4 * h->dim, true);
+    if (st != LQG_OK) return st;
+    st = host_rows(h, h->h2_off_ids, h->nbr.data(), 4 * h->deg, true);
+    if (st != LQG_OK) return st;
+    LQG_HIP_TRY(hipMemcpy(h->d_h2signs.p, h->h2_signs.data(), 4ull * h->h2_p,
+                          hipMemcpyHostToDevice));
     return LQG_OK;
 }
 
@@ -140,7 +162,7 @@ void h2_reset(lqg_index* h) {
 
 lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
                         const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, uint32_t* d_results,
-                        float* d_dists, uint32_t* d_stats, hipStream_t s) {
+                        float* d_dists, uint32_t* d_stats, hipStream_t s, Scratch* sc) {
     if (!h->h2_ready) return fail(LQG_ERR_STATE, "FastScan mode needs a sidecar");
     const int cls = ef_class(ef_max);
     if (cls == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported on the device");
@@ -156,9 +178,9 @@ lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const
     const int groups = h2_groups(ix, cfg, kmax, static_cast<uint32_t>(nq));
     if (groups <= 0) return fail(LQG_ERR_DEVICE, "FastScan: no launch geometry");
     cfg.groups = groups;
-    LQG_HIP_TRY(h->d_h2gvis.reserve((static_cast<size_t>(groups) * cfg.waves_per_group) << kGvisBits));
-    LQG_HIP_TRY(h->d_h2err.reserve(1));
-    LQG_HIP_TRY(h->d_counter.reserve(1));
+    LQG_HIP_TRY(sc->gvis.reserve((static_cast<size_t>(groups) * cfg.waves_per_group) << kGvisBits));
+    LQG_HIP_TRY(sc->err.reserve(1));
+    LQG_HIP_TRY(sc->counter.reserve(1));
     H2Job job;
     job.queries = d_queries;
     job.ef = d_ef;
@@ -170,20 +192,21 @@ lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const
     job.out_ids = d_results;
     job.out_d = d_dists;
     job.out_stats = d_stats;
-    job.counter = h->d_counter.p;
-    job.gvis = h->d_h2gvis.p;
+    job.counter = sc->counter.p;
+    job.gvis = sc->gvis.p;
     job.gvis_bits = kGvisBits;
-    job.err = h->d_h2err.p;
-    LQG_HIP_TRY(launch_reset_counter(h->d_counter.p, s));
-    LQG_HIP_TRY(launch_reset_counter(h->d_h2err.p, s));
+    job.err = sc->err.p;
+    LQG_HIP_TRY(launch_reset_counter(sc->counter.p, s));
+    LQG_HIP_TRY(launch_reset_counter(sc->err.p, s));
     LQG_HIP_TRY(h2_launch_search(ix, job, cfg, s));
     return LQG_OK;
 }
 
-lqg_status h2_check_error(lqg_index* h) {
-    if (h->mode != LQG_MODE_FASTSCAN || !h->d_h2err.p) return LQG_OK;
+lqg_status h2_check_error(lqg_index* h, Scratch* sc) {
+    if (!sc) sc = &h->scratch;
+    if (h->mode != LQG_MODE_FASTSCAN || !sc->err.p) return LQG_OK;
     uint32_t e = 0;
-    LQG_HIP_TRY(hipMemcpy(&e, h->d_h2err.p, 4, hipMemcpyDeviceToHost));
+    LQG_HIP_TRY(hipMemcpy(&e, sc->err.p, 4, hipMemcpyDeviceToHost));
     if (e) return fail(LQG_ERR_UNSUPPORTED, "FastScan: a query's visited set overflowed");
     return LQG_OK;
 }
@@ -264,32 +287,30 @@ lqg_status lqg_h2_set_sidecar(lqg_index* h, const float* signs, const uint8_t* c
     st = lqg::alloc_records(h);
     if (st != LQG_OK) return st;
     if (layout == 1) {
-        LQG_HIP_TRY(hipMemcpy2D(h->d_h2rec.p + h->h2_off_codes, h->h2_stride, codes, cb, cb, h->n,
-                                hipMemcpyHostToDevice));
+        st = lqg::host_rows(h, h->h2_off_codes, const_cast<uint8_t*>(codes), cb, true);
     } else {
-        // block layout -> vector-major, a chunk of nodes at a time
-        const uint64_t chunk = std::max<uint64_t>(1, (32ull << 20) / cb);
-        std::vector<uint8_t> vm(std::min<uint64_t>(chunk, h->n) * cb);
-        for (uint64_t s = 0; s < h->n; s += chunk) {
-            const uint64_t e = std::min<uint64_t>(h->n, s + chunk);
-            std::fill(vm.begin(), vm.end(), 0);
-            for (uint64_t u = s; u < e; ++u) {
-                const uint8_t* src = codes + u * cb;
-                uint8_t* dst = vm.data() + (u - s) * cb;
-                for (uint32_t v = 0; v < deg; ++v)
-                    for (uint32_t c = 0; c < p / 4; ++c) {
-                        const uint32_t nib = lqg::block_nibble(src, p, v, c);
-                        dst[static_cast<size_t>(v) * (p / 8) + c / 2] |=
-                            static_cast<uint8_t>((c & 1u) ? nib << 4 : nib);
-                    }
-            }
-            LQG_HIP_TRY(hipMemcpy2D(h->d_h2rec.p + s * h->h2_stride + h->h2_off_codes,
-                                    h->h2_stride, vm.data(), cb, cb, e - s,
-                                    hipMemcpyHostToDevice));
+        // block layout -> vector-major
+        std::vector<uint8_t> vm;
+        try {
+            vm.assign(h->n * cb, 0);
+        } catch (...) {
+            return fail(LQG_ERR_NOMEM, "host allocation failed");
         }
+        for (uint64_t u = 0; u < h->n; ++u) {
+            const uint8_t* src = codes + u * cb;
+            uint8_t* dst = vm.data() + u * cb;
+            for (uint32_t v = 0; v < deg; ++v)
+                for (uint32_t c = 0; c < p / 4; ++c) {
+                    const uint32_t nib = lqg::block_nibble(src, p, v, c);
+                    dst[static_cast<size_t>(v) * (p / 8) + c / 2] |=
+                        static_cast<uint8_t>((c & 1u) ? nib << 4 : nib);
+                }
+        }
+        st = lqg::host_rows(h, h->h2_off_codes, vm.data(), cb, true);
     }
-    LQG_HIP_TRY(hipMemcpy2D(h->d_h2rec.p + h->h2_off_fac, h->h2_stride, factors, 12ull * deg,
-                            12ull * deg, h->n, hipMemcpyHostToDevice));
+    if (st != LQG_OK) return st;
+    st = lqg::host_rows(h, h->h2_off_fac, const_cast<float*>(factors), 12ull * deg, true);
+    if (st != LQG_OK) return st;
     h->h2_ready = true;
     return LQG_OK;
 }
@@ -300,12 +321,15 @@ lqg_status lqg_h2_get_sidecar(const lqg_index* h, float* signs, uint8_t* codes, 
     const uint32_t p = h->h2_p, deg = static_cast<uint32_t>(h->deg);
     const size_t cb = static_cast<size_t>(deg) * (p / 8);
     if (signs) memcpy(signs, h->h2_signs.data(), 4ull * p);
-    if (codes)
-        LQG_HIP_TRY(hipMemcpy2D(codes, cb, h->d_h2rec.p + h->h2_off_codes, h->h2_stride, cb, h->n,
-                                hipMemcpyDeviceToHost));
-    if (factors)
-        LQG_HIP_TRY(hipMemcpy2D(factors, 12ull * deg, h->d_h2rec.p + h->h2_off_fac, h->h2_stride,
-                                12ull * deg, h->n, hipMemcpyDeviceToHost));
+    lqg_index* hm = const_cast<lqg_index*>(h);  // the copies use the handle's stream
+    if (codes) {
+        lqg_status st = lqg::host_rows(hm, h->h2_off_codes, codes, cb, false);
+        if (st != LQG_OK) return st;
+    }
+    if (factors) {
+        lqg_status st = lqg::host_rows(hm, h->h2_off_fac, factors, 12ull * deg, false);
+        if (st != LQG_OK) return st;
+    }
     return LQG_OK;
 }
 
@@ -327,23 +351,24 @@ lqg_status lqg_h2_save_sidecar(const lqg_index* h, const char* path) {
     hd.fingerprint = lqg::fingerprint(h);
     bool ok = fwrite(&hd, sizeof hd, 1, f) == 1;
     ok = ok && fwrite(h->h2_signs.data(), 4, p, f) == p;
-    const uint64_t chunk = std::max<uint64_t>(1, (64ull << 20) / (cb + fb));
-    std::vector<uint8_t> c(std::min<uint64_t>(chunk, h->n) * cb);
-    std::vector<uint8_t> fc(std::min<uint64_t>(chunk, h->n) * fb);
     // codes of every node, then factors of every node
     for (int part = 0; ok && part < 2; ++part) {
         const size_t w = part == 0 ? cb : fb;
-        const uint32_t off = part == 0 ? h->h2_off_codes : h->h2_off_fac;
-        std::vector<uint8_t>& buf = part == 0 ? c : fc;
-        for (uint64_t s = 0; ok && s < h->n; s += chunk) {
-            const uint64_t e = std::min<uint64_t>(h->n, s + chunk);
-            if (hipMemcpy2D(buf.data(), w, h->d_h2rec.p + s * h->h2_stride + off, h->h2_stride, w,
-                            e - s, hipMemcpyDeviceToHost) != hipSuccess) {
-                fclose(f);
-                return fail(LQG_ERR_DEVICE, "sidecar download failed");
-            }
-            ok = fwrite(buf.data(), 1, (e - s) * w, f) == (e - s) * w;
+        std::vector<uint8_t> buf;
+        try {
+            buf.resize(h->n * w);
+        } catch (...) {
+            fclose(f);
+            return fail(LQG_ERR_NOMEM, "host allocation failed");
         }
+        lqg_status st = lqg::host_rows(const_cast<lqg_index*>(h),
+                                       part == 0 ? h->h2_off_codes : h->h2_off_fac, buf.data(), w,
+                                       false);
+        if (st != LQG_OK) {
+            fclose(f);
+            return st;
+        }
+        ok = fwrite(buf.data(), 1, buf.size(), f) == buf.size();
     }
     ok = (fclose(f) == 0) && ok;
     return ok ? LQG_OK : fail(LQG_ERR_IO, std::string("write failed: ") + path);
@@ -380,18 +405,22 @@ lqg_status lqg_h2_load_sidecar(lqg_index* h, const char* path) {
             return st;
         }
     }
-    const uint64_t chunk = std::max<uint64_t>(1, (64ull << 20) / (cb + fb));
-    std::vector<uint8_t> buf(std::min<uint64_t>(chunk, h->n) * std::max(cb, fb));
     for (int part = 0; ok && part < 2; ++part) {
         const size_t w = part == 0 ? cb : fb;
-        const uint32_t off = part == 0 ? h->h2_off_codes : h->h2_off_fac;
-        for (uint64_t s = 0; ok && s < h->n; s += chunk) {
-            const uint64_t e = std::min<uint64_t>(h->n, s + chunk);
-            ok = fread(buf.data(), 1, (e - s) * w, f) == (e - s) * w;
-            if (ok && hipMemcpy2D(h->d_h2rec.p + s * h->h2_stride + off, h->h2_stride, buf.data(),
-                                  w, w, e - s, hipMemcpyHostToDevice) != hipSuccess) {
+        std::vector<uint8_t> buf;
+        try {
+            buf.resize(h->n * w);
+        } catch (...) {
+            fclose(f);
+            return fail(LQG_ERR_NOMEM, "host allocation failed");
+        }
+        ok = fread(buf.data(), 1, buf.size(), f) == buf.size();
+        if (ok) {
+            st = lqg::host_rows(h, part == 0 ? h->h2_off_codes : h->h2_off_fac, buf.data(), w,
+                                true);
+            if (st != LQG_OK) {
                 fclose(f);
-                return fail(LQG_ERR_DEVICE, "sidecar upload failed");
+                return st;
             }
         }
     }

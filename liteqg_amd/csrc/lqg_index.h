@@ -43,6 +43,19 @@ struct DevBuf {
     }
 };
 
+// Device scratch of one search launch.  Launches that may run at the same
+// time (different streams, graph replays, serving lanes) each own one.
+struct Scratch {
+    DevBuf<uint32_t> counter;  // work-queue head
+    DevBuf<uint32_t> err;      // FastScan visited-set overflow flag
+    DevBuf<uint32_t> gvis;     // FastScan per-wave overflow visited tables
+    void release() {
+        counter.release();
+        err.release();
+        gvis.release();
+    }
+};
+
 // Smallest pool-capacity class that holds ef, 0 if none does.
 int ef_class(uint32_t ef);
 
@@ -77,7 +90,8 @@ struct lqg_index {
     lqg::DevBuf<float> d_min, d_scale;
     // per-call scratch (grow only)
     lqg::DevBuf<float> d_q;
-    lqg::DevBuf<uint32_t> d_ids, d_ef, d_k, d_stats, d_counter;
+    lqg::DevBuf<uint32_t> d_ids, d_ef, d_k, d_stats;
+    lqg::Scratch scratch;  // for the entry points that run on the handle's stream
     lqg::DevBuf<float> d_dist;
     lqg::DevBuf<uint32_t> d_visit;  // exact visit counting bitmaps
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -93,7 +107,6 @@ struct lqg_index {
     std::vector<float> h2_signs;
     lqg::DevBuf<uint8_t> d_h2rec;
     lqg::DevBuf<float> d_h2signs;
-    lqg::DevBuf<uint32_t> d_h2gvis, d_h2err;
     lqg::H2Cfg h2_launch[5];
     bool h2_launch_set[5] = {false, false, false, false, false};
 
@@ -133,16 +146,17 @@ namespace lqg {
 // Checks the device, creates the stream and uploads the host copy (once).
 lqg_status ensure_device(lqg_index* h);
 
-// Enqueues one search launch over device arrays.
+// Enqueues one search launch over device arrays, in the handle's mode, with
+// scratch `sc` (null: the handle's own).
 lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
                      const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, lqg_order order,
                      uint32_t* d_results, float* d_dists, uint32_t* d_stats, hipStream_t s,
-                     bool count_visits = false);
+                     bool count_visits = false, Scratch* sc = nullptr);
 
 // FastScan counterpart of run_batch (lqg_h2.cpp); d_stats gets (E, pops).
 lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
                         const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, uint32_t* d_results,
-                        float* d_dists, uint32_t* d_stats, hipStream_t s);
+                        float* d_dists, uint32_t* d_stats, hipStream_t s, Scratch* sc);
 
 // Launch settings of the FastScan kernel used when none were set for a class.
 H2Cfg h2_default_launch(int ef_cap);
@@ -150,8 +164,9 @@ H2Cfg h2_default_launch(int ef_cap);
 // Frees the FastScan records (index reloaded or rebuilt).
 void h2_reset(lqg_index* h);
 
-// After a synchronised FastScan launch: fails if a visited set overflowed.
-lqg_status h2_check_error(lqg_index* h);
+// After a synchronised FastScan launch with scratch sc (null: the handle's):
+// fails if a visited set overflowed.
+lqg_status h2_check_error(lqg_index* h, Scratch* sc = nullptr);
 
 // Device graph build for lqg_build_index.
 lqg_status build_index(lqg_index* h, const float* data, uint32_t ef_build, uint32_t num_iter);

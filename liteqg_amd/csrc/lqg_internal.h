@@ -100,8 +100,9 @@ hipError_t h2_launch_probe(const H2Dev& ix, const float* queries, uint32_t nq,
 // Sidecar build: rotates every raw vector, then fills codes and factors of
 // every record (raw and ids must already be in the records).
 hipError_t h2_launch_sidecar(const H2Dev& ix, float* rot_scratch, hipStream_t stream);
-// Copies raw rows (n x dim f32) and neighbour ids (n x deg) into the records.
-hipError_t h2_launch_pack(const H2Dev& ix, const float* raw, const uint32_t* nbr,
+// Copies w bytes per node between a contiguous device array (n x w) and
+// byte offset `off` of every record (to_rec: into the records).
+hipError_t h2_launch_rows(const H2Dev& ix, uint32_t off, void* buf, uint32_t w, bool to_rec,
                           hipStream_t stream);
 
 // Enqueues the search kernel (queries pulled from job.counter).

@@ -1,0 +1,435 @@
+// lqg_serve.cpp -- serving engine: single-query requests, continuously
+// batched over several batch launches in flight on separate streams.
+//
+// Requests live in a fixed table of slots (query in pinned host memory, ef,
+// k, results).  One dispatcher thread loops: it retires launches whose
+// completion event has fired (copying each request's results to its slot),
+// and whenever a lane (stream + device buffers + launch scratch) is idle and
+// requests are pending, it gathers up to max_batch of them into the lane's
+// pinned staging buffer and enqueues copy-in, the search launch and
+// copy-out on the lane's stream.  A request arriving while launches run is
+// therefore admitted to the next free lane without waiting for the others;
+// with several lanes busy, several batches occupy the device at once.  Each
+// query is walked by one wavefront independently of the others, so a
+// request's results do not depend on its batch.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "lqg_index.h"
+
+namespace {
+
+using lqg::fail;
+
+int64_t now_ns() {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+enum SlotState : int { kFree = 0, kPending = 1, kRunning = 2, kDone = 3, kFailed = 4 };
+
+struct Lane {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    lqg::Scratch scratch;
+    lqg::DevBuf<float> d_q, d_dist;
+    lqg::DevBuf<uint32_t> d_ef, d_k, d_ids;
+    float* h_q = nullptr;      // pinned staging: max_batch x dim
+    uint32_t* h_ek = nullptr;  // pinned: ef[max_batch] | k[max_batch]
+    uint32_t* h_ids = nullptr; // pinned: max_batch x kmax
+    float* h_dist = nullptr;   // pinned: max_batch x kmax
+    uint32_t* h_err = nullptr; // pinned: FastScan overflow flag
+    std::vector<uint32_t> slots;
+    bool busy = false;
+    bool fastscan = false;
+    bool failed = false;
+};
+
+}  // namespace
+
+struct lqg_server {
+    lqg_index* h = nullptr;
+    lqg_serve_config cfg{};
+    uint32_t dim = 0;
+    int device = 0;
+    float* q_store = nullptr;  // pinned: slots x dim
+    std::vector<uint32_t> ef, k, gen, ids;
+    std::vector<float> dist;
+    std::vector<int64_t> t_done;
+    std::unique_ptr<std::atomic<int>[]> state;
+    std::vector<uint32_t> free_list;
+    std::deque<uint32_t> pending;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::vector<Lane> lanes;
+    std::thread worker;
+    bool stop = false;
+    lqg_serve_stats st{};
+};
+
+namespace {
+
+void free_lane(Lane& l) {
+    if (l.stream) (void)hipStreamSynchronize(l.stream);
+    l.scratch.release();
+    l.d_q.release();
+    l.d_dist.release();
+    l.d_ef.release();
+    l.d_k.release();
+    l.d_ids.release();
+    if (l.h_q) (void)hipHostFree(l.h_q);
+    if (l.h_ek) (void)hipHostFree(l.h_ek);
+    if (l.h_ids) (void)hipHostFree(l.h_ids);
+    if (l.h_dist) (void)hipHostFree(l.h_dist);
+    if (l.h_err) (void)hipHostFree(l.h_err);
+    if (l.done) (void)hipEventDestroy(l.done);
+    if (l.stream) (void)hipStreamDestroy(l.stream);
+    l = Lane();
+}
+
+void destroy(lqg_server* s) {
+    for (Lane& l : s->lanes) free_lane(l);
+    if (s->q_store) (void)hipHostFree(s->q_store);
+    delete s;
+}
+
+// Enqueues one batch on an idle lane (dispatcher thread, no lock held: the
+// batch's slots are in state kRunning, which nobody else touches).
+void launch(lqg_server* s, Lane& l, const std::vector<uint32_t>& batch) {
+    const uint32_t b = static_cast<uint32_t>(batch.size()), d = s->dim, mb = s->cfg.max_batch;
+    for (uint32_t i = 0; i < b; ++i) {
+        const uint32_t slot = batch[i];
+        memcpy(l.h_q + static_cast<size_t>(i) * d, s->q_store + static_cast<size_t>(slot) * d,
+               4ull * d);
+        l.h_ek[i] = s->ef[slot];
+        l.h_ek[mb + i] = s->k[slot];
+    }
+    l.slots = batch;
+    l.failed = false;
+    l.fastscan = s->h->mode == LQG_MODE_FASTSCAN;
+    bool ok = hipMemcpyAsync(l.d_q.p, l.h_q, 4ull * b * d, hipMemcpyHostToDevice, l.stream) ==
+                  hipSuccess &&
+              hipMemcpyAsync(l.d_ef.p, l.h_ek, 4ull * b, hipMemcpyHostToDevice, l.stream) ==
+                  hipSuccess &&
+              hipMemcpyAsync(l.d_k.p, l.h_ek + mb, 4ull * b, hipMemcpyHostToDevice, l.stream) ==
+                  hipSuccess;
+    ok = ok && lqg::run_batch(s->h, l.d_q.p, b, l.d_ef.p, l.d_k.p, s->cfg.ef_max, s->cfg.kmax,
+                              s->cfg.order, l.d_ids.p, l.d_dist.p, nullptr, l.stream, false,
+                              &l.scratch) == LQG_OK;
+    const size_t rb = 4ull * b * s->cfg.kmax;
+    ok = ok && hipMemcpyAsync(l.h_ids, l.d_ids.p, rb, hipMemcpyDeviceToHost, l.stream) == hipSuccess &&
+         hipMemcpyAsync(l.h_dist, l.d_dist.p, rb, hipMemcpyDeviceToHost, l.stream) == hipSuccess;
+    if (ok && l.fastscan)
+        ok = hipMemcpyAsync(l.h_err, l.scratch.err.p, 4, hipMemcpyDeviceToHost, l.stream) ==
+             hipSuccess;
+    ok = ok && hipEventRecord(l.done, l.stream) == hipSuccess;
+    l.failed = !ok;
+    l.busy = true;
+}
+
+// Copies a finished lane's results to its requests' slots.
+void retire(lqg_server* s, Lane& l) {
+    // a launch that failed to enqueue may still have copies in flight that
+    // read the lane's staging buffers
+    if (l.failed) (void)hipStreamSynchronize(l.stream);
+    const bool bad = l.failed || (l.fastscan && l.h_err[0] != 0);
+    const uint32_t km = s->cfg.kmax;
+    const int64_t t = now_ns();
+    std::lock_guard<std::mutex> g(s->mu);
+    for (size_t i = 0; i < l.slots.size(); ++i) {
+        const uint32_t slot = l.slots[i];
+        if (!bad) {
+            memcpy(&s->ids[static_cast<size_t>(slot) * km], l.h_ids + i * km, 4ull * km);
+            memcpy(&s->dist[static_cast<size_t>(slot) * km], l.h_dist + i * km, 4ull * km);
+        }
+        s->t_done[slot] = t;
+        s->state[slot].store(bad ? kFailed : kDone, std::memory_order_release);
+    }
+    s->st.completed += l.slots.size();
+    l.slots.clear();
+    l.busy = false;
+    s->cv_done.notify_all();
+}
+
+void dispatcher(lqg_server* s) {
+    (void)hipSetDevice(s->device);
+    std::vector<uint32_t> batch;
+    for (;;) {
+        bool progressed = false;
+        uint32_t busy = 0;
+        for (Lane& l : s->lanes) {
+            if (!l.busy) continue;
+            if (l.failed || hipEventQuery(l.done) == hipSuccess) {
+                retire(s, l);
+                progressed = true;
+            } else {
+                ++busy;
+            }
+        }
+        Lane* idle = nullptr;
+        for (Lane& l : s->lanes)
+            if (!l.busy) {
+                idle = &l;
+                break;
+            }
+        batch.clear();
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (s->stop && s->pending.empty() && busy == 0) break;
+            if (idle && !s->pending.empty()) {
+                while (!s->pending.empty() && batch.size() < s->cfg.max_batch) {
+                    const uint32_t slot = s->pending.front();
+                    s->pending.pop_front();
+                    s->state[slot].store(kRunning, std::memory_order_relaxed);
+                    batch.push_back(slot);
+                }
+                ++s->st.batches;
+                s->st.max_batch_seen =
+                    std::max(s->st.max_batch_seen, static_cast<uint32_t>(batch.size()));
+                s->st.max_lanes_busy = std::max(s->st.max_lanes_busy, busy + 1);
+            } else if (!progressed && busy == 0) {
+                s->cv_work.wait(lk, [s] { return s->stop || !s->pending.empty(); });
+                continue;
+            }
+        }
+        if (!batch.empty()) {
+            launch(s, *idle, batch);
+            continue;
+        }
+        if (!progressed) std::this_thread::yield();
+    }
+}
+
+lqg_status poll_impl(lqg_server* s, uint64_t ticket, uint32_t* results, float* dists,
+                     bool block, int64_t* t_done) {
+    const uint32_t slot = static_cast<uint32_t>(ticket & 0xFFFFFFFFu);
+    const uint32_t g = static_cast<uint32_t>(ticket >> 32);
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (slot >= s->cfg.slots || s->gen[slot] != g || s->state[slot].load() == kFree)
+        return fail(LQG_ERR_ARG, "unknown or released ticket");
+    for (;;) {
+        const int st = s->state[slot].load(std::memory_order_acquire);
+        if (st == kDone || st == kFailed) {
+            if (st == kDone) {
+                const uint32_t km = s->cfg.kmax, kq = s->k[slot];
+                if (results) memcpy(results, &s->ids[static_cast<size_t>(slot) * km], 4ull * kq);
+                if (dists) memcpy(dists, &s->dist[static_cast<size_t>(slot) * km], 4ull * kq);
+            }
+            if (t_done) *t_done = s->t_done[slot];
+            s->state[slot].store(kFree, std::memory_order_relaxed);
+            s->free_list.push_back(slot);
+            return st == kDone ? LQG_OK
+                               : fail(LQG_ERR_DEVICE, "serving launch failed (or a FastScan "
+                                                      "visited set overflowed)");
+        }
+        if (!block) return LQG_ERR_BUSY;
+        s->cv_done.wait(lk);
+    }
+}
+
+lqg_status submit_impl(lqg_server* s, const float* query, uint32_t ef, uint32_t k,
+                       uint64_t* ticket) {
+    if (ef == 0 || ef > s->cfg.ef_max) return fail(LQG_ERR_ARG, "ef must be in [1, ef_max]");
+    if (k == 0 || k > s->cfg.kmax) return fail(LQG_ERR_ARG, "k must be in [1, kmax]");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->stop) return fail(LQG_ERR_STATE, "server is stopping");
+    if (s->free_list.empty()) return LQG_ERR_BUSY;
+    const uint32_t slot = s->free_list.back();
+    s->free_list.pop_back();
+    ++s->gen[slot];
+    memcpy(s->q_store + static_cast<size_t>(slot) * s->dim, query, 4ull * s->dim);
+    s->ef[slot] = ef;
+    s->k[slot] = k;
+    s->state[slot].store(kPending, std::memory_order_relaxed);
+    s->pending.push_back(slot);
+    ++s->st.submitted;
+    *ticket = (static_cast<uint64_t>(s->gen[slot]) << 32) | slot;
+    lk.unlock();
+    s->cv_work.notify_one();
+    return LQG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+lqg_status lqg_serve_start(lqg_index* h, const lqg_serve_config* cfg, lqg_server** out) {
+    if (!h || !cfg || !out) return fail(LQG_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->lanes < 1 || cfg->lanes > 16) return fail(LQG_ERR_ARG, "lanes must be in [1, 16]");
+    if (cfg->max_batch < 1 || cfg->max_batch > 65536)
+        return fail(LQG_ERR_ARG, "max_batch must be in [1, 65536]");
+    if (cfg->slots < cfg->max_batch) return fail(LQG_ERR_ARG, "slots must be >= max_batch");
+    if (cfg->kmax == 0) return fail(LQG_ERR_ARG, "kmax must be positive");
+    if (cfg->ef_max == 0 || lqg::ef_class(cfg->ef_max) == 0)
+        return fail(LQG_ERR_ARG, "ef_max must be in [1, 1024]");
+    if (cfg->order != LQG_ORDER_REFERENCE && cfg->order != LQG_ORDER_NEAREST)
+        return fail(LQG_ERR_ARG, "unknown order");
+    lqg_status st = lqg::ensure_device(h);
+    if (st != LQG_OK) return st;
+    lqg_server* s = new (std::nothrow) lqg_server();
+    if (!s) return fail(LQG_ERR_NOMEM, "allocation failed");
+    s->h = h;
+    s->cfg = *cfg;
+    s->dim = static_cast<uint32_t>(h->dim);
+    (void)hipGetDevice(&s->device);
+    const size_t slots = cfg->slots, km = cfg->kmax, mb = cfg->max_batch, d = s->dim;
+    try {
+        s->ef.assign(slots, 0);
+        s->k.assign(slots, 0);
+        s->gen.assign(slots, 0);
+        s->ids.assign(slots * km, 0);
+        s->dist.assign(slots * km, 0.0f);
+        s->t_done.assign(slots, 0);
+        s->state.reset(new std::atomic<int>[slots]);
+        s->free_list.reserve(slots);
+        s->lanes.resize(cfg->lanes);
+    } catch (...) {
+        delete s;
+        return fail(LQG_ERR_NOMEM, "host allocation failed");
+    }
+    for (size_t i = 0; i < slots; ++i) {
+        s->state[i].store(kFree);
+        s->free_list.push_back(static_cast<uint32_t>(slots - 1 - i));
+    }
+    hipError_t e = hipHostMalloc(reinterpret_cast<void**>(&s->q_store), 4 * slots * d);
+    for (Lane& l : s->lanes) {
+        if (e != hipSuccess) break;
+        e = hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&l.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = l.d_q.reserve(mb * d);
+        if (e == hipSuccess) e = l.d_dist.reserve(mb * km);
+        if (e == hipSuccess) e = l.d_ids.reserve(mb * km);
+        if (e == hipSuccess) e = l.d_ef.reserve(mb);
+        if (e == hipSuccess) e = l.d_k.reserve(mb);
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&l.h_q), 4 * mb * d);
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&l.h_ek), 8 * mb);
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&l.h_ids), 4 * mb * km);
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&l.h_dist), 4 * mb * km);
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&l.h_err), 4);
+        if (e == hipSuccess) l.h_err[0] = 0;
+    }
+    if (e != hipSuccess) {
+        destroy(s);
+        return lqg::hip_fail(e, "serving buffers");
+    }
+    try {
+        s->worker = std::thread(dispatcher, s);
+    } catch (...) {
+        destroy(s);
+        return fail(LQG_ERR_NOMEM, "cannot start the dispatcher thread");
+    }
+    *out = s;
+    return LQG_OK;
+}
+
+lqg_status lqg_serve_submit(lqg_server* s, const float* query, uint32_t ef, uint32_t k,
+                            uint64_t* ticket) {
+    if (!s || !query || !ticket) return fail(LQG_ERR_ARG, "null argument");
+    for (uint32_t i = 0; i < s->dim; ++i)
+        if (!std::isfinite(query[i])) return fail(LQG_ERR_ARG, "query contains a non-finite value");
+    return submit_impl(s, query, ef, k, ticket);
+}
+
+lqg_status lqg_serve_poll(lqg_server* s, uint64_t ticket, uint32_t* results, float* dists) {
+    if (!s) return fail(LQG_ERR_ARG, "null server");
+    return poll_impl(s, ticket, results, dists, false, nullptr);
+}
+
+lqg_status lqg_serve_wait(lqg_server* s, uint64_t ticket, uint32_t* results, float* dists) {
+    if (!s) return fail(LQG_ERR_ARG, "null server");
+    return poll_impl(s, ticket, results, dists, true, nullptr);
+}
+
+lqg_status lqg_serve_get_stats(lqg_server* s, lqg_serve_stats* out) {
+    if (!s || !out) return fail(LQG_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    *out = s->st;
+    return LQG_OK;
+}
+
+lqg_status lqg_serve_stop(lqg_server* s) {
+    if (!s) return LQG_OK;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    destroy(s);
+    return LQG_OK;
+}
+
+lqg_status lqg_serve_replay(lqg_server* s, const float* queries, uint64_t nq, const uint32_t* ef,
+                            const uint32_t* k, const uint64_t* arrival_ns, uint64_t* latency_ns,
+                            uint32_t* results) {
+    if (!s || !queries || !ef || !k || !arrival_ns || !latency_ns)
+        return fail(LQG_ERR_ARG, "null argument");
+    if (nq == 0) return fail(LQG_ERR_ARG, "nq must be positive");
+    for (uint64_t i = 0; i < nq; ++i) {
+        if (ef[i] == 0 || ef[i] > s->cfg.ef_max || k[i] == 0 || k[i] > s->cfg.kmax)
+            return fail(LQG_ERR_ARG, "ef / k out of the server's range");
+        if (i > 0 && arrival_ns[i] < arrival_ns[i - 1])
+            return fail(LQG_ERR_ARG, "arrival times must be non-decreasing");
+    }
+    for (uint64_t i = 0; i < nq * s->dim; ++i)
+        if (!std::isfinite(queries[i])) return fail(LQG_ERR_ARG, "query contains a non-finite value");
+    const uint32_t km = s->cfg.kmax;
+    std::vector<uint64_t> tickets(nq);
+    std::unique_ptr<std::atomic<int>[]> ready(new std::atomic<int>[nq]);
+    for (uint64_t i = 0; i < nq; ++i) ready[i].store(0);
+    std::atomic<int> failed{0};
+    const int64_t t0 = now_ns();
+    // collector: waits for the tickets in submission order
+    std::thread collector([&] {
+        std::vector<uint32_t> buf(km);
+        for (uint64_t i = 0; i < nq; ++i) {
+            while (!ready[i].load(std::memory_order_acquire)) std::this_thread::yield();
+            if (ready[i].load() < 0) {
+                failed.store(1);
+                latency_ns[i] = 0;
+                continue;
+            }
+            int64_t td = 0;
+            uint32_t* out = results ? results + i * km : buf.data();
+            if (poll_impl(s, tickets[i], out, nullptr, true, &td) != LQG_OK) failed.store(1);
+            if (results)
+                for (uint32_t j = k[i]; j < km; ++j) out[j] = 0xFFFFFFFFu;
+            const int64_t sched = t0 + static_cast<int64_t>(arrival_ns[i]);
+            latency_ns[i] = static_cast<uint64_t>(td > sched ? td - sched : 0);
+        }
+    });
+    for (uint64_t i = 0; i < nq; ++i) {
+        const int64_t at = t0 + static_cast<int64_t>(arrival_ns[i]);
+        while (now_ns() < at) {
+        }
+        for (;;) {
+            const lqg_status st = submit_impl(s, queries + i * s->dim, ef[i], k[i], &tickets[i]);
+            if (st == LQG_OK) {
+                ready[i].store(1, std::memory_order_release);
+                break;
+            }
+            if (st != LQG_ERR_BUSY) {
+                ready[i].store(-1, std::memory_order_release);
+                break;
+            }
+            std::this_thread::yield();
+        }
+    }
+    collector.join();
+    if (failed.load()) return fail(LQG_ERR_DEVICE, "some replayed requests failed");
+    return LQG_OK;
+}
+
+}  // extern "C"

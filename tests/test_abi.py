@@ -154,3 +154,47 @@ def test_build_index_shape_check():
         ix.build_index(np.zeros((99, 16), np.float32), 50)
     with pytest.raises(ValueError, match="shape"):
         ix.build_index(np.zeros((2, 3, 4), np.float32), 50)
+
+
+def test_fastscan_graph_and_serving_validation():
+    """Argument and call-order checks of the FastScan, graph and serving entry
+    points fire before any device work."""
+    import ctypes as C
+
+    from liteqg_amd import _lib
+    from liteqg_amd.symphonyqg import Index
+
+    L = _lib.lib()
+    ix = Index("QG", "L2", 100, 64, 32)
+    # FastScan mode needs a sidecar
+    assert L.lqg_set_mode(ix._h, 1) == 2
+    assert L.lqg_set_mode(ix._h, 5) == 1
+    with pytest.raises(ValueError):
+        ix.set_mode("bogus")
+    # sidecar entry points need a loaded index
+    assert L.lqg_h2_build_sidecar(ix._h, None, 0) == 2
+    assert L.lqg_h2_get_sidecar(ix._h, None, None, None) == 2
+    assert L.lqg_h2_save_sidecar(ix._h, b"/nonexistent/x") == 2
+    q = np.zeros(64, np.float32)
+    lut = np.zeros(256, np.uint8)
+    lw = np.zeros(2, np.float32)
+    sq = np.zeros(1, np.int32)
+    assert L.lqg_h2_probe(ix._h, _lib.ptr(q), 0, None, _lib.ptr(lut), _lib.ptr(lw),
+                          _lib.ptr(sq), None, None) == 1
+    # graph capture: null output, zero queries, unknown order
+    g = C.c_void_p()
+    assert L.lqg_graph_create(ix._h, None, 1, None, None, 10, 10, 1, None, None, None,
+                              C.byref(g)) == 1
+    assert L.lqg_graph_create(ix._h, C.c_void_p(16), 0, None, None, 10, 10, 1, C.c_void_p(16),
+                              None, None, C.byref(g)) == 1
+    assert L.lqg_graph_launch(None, None) == 1
+    # serving configuration bounds
+    s = C.c_void_p()
+    for cfg in [(0, 8, 8, 64, 10, 1), (17, 8, 8, 64, 10, 1), (2, 8, 4, 64, 10, 1),
+                (2, 8, 8, 0, 10, 1), (2, 8, 8, 2048, 10, 1), (2, 8, 8, 64, 0, 1),
+                (2, 8, 8, 64, 10, 7)]:
+        assert L.lqg_serve_start(ix._h, C.byref(_lib.ServeConfig(*cfg)), C.byref(s)) == 1, cfg
+    # a valid configuration on an index with no data fails on state, not args
+    assert L.lqg_serve_start(ix._h, C.byref(_lib.ServeConfig(2, 8, 8, 64, 10, 1)),
+                             C.byref(s)) == 2
+    assert L.lqg_serve_stop(None) == 0
