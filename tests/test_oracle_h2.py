@@ -141,6 +141,33 @@ def test_scan_estimates_match_reference(oracle_mod, name):
         assert np.array_equal(est.view(np.uint32), h["prep_est"][i].view(np.uint32))
 
 
+def test_estimate_order_is_decided_by_goldens(oracle_mod):
+    """The source order (fac_dq * width) * f differs from the reference build's
+    (f * fac_dq) * width on some golden estimate, so the fixtures decide."""
+    f32 = np.float32
+    differs = 0
+    for name in H2_NAMES:
+        ix, h = _h2(name)
+        p = oracle_mod.padded_dim(ix.dim)
+        fac = h["factors"][ix.ep].reshape(3, ix.deg)
+        for i in range(h["prep_rq"].shape[0]):
+            sums, est = oracle_mod.scan_estimates(
+                h["codes"][ix.ep], 0, h["factors"][ix.ep], ix.deg, p, h["prep_lut"][i],
+                float(h["prep_sqr_y"][i]), float(h["prep_lo"][i]), float(h["prep_width"][i]),
+                int(h["prep_sumq"][i]))
+            f = (sums.astype(np.uint16).view(np.int16).astype(np.int32) * 2 -
+                 int(h["prep_sumq"][i])).astype(f32)
+            w, lo = h["prep_width"][i], h["prep_lo"][i]
+            a = (fac[0] + h["prep_sqr_y"][i]).astype(f32)
+            # fma(fac_vq, lo, a): the product is exact in double
+            b = (fac[2].astype(np.float64) * np.float64(lo) + a.astype(np.float64)).astype(f32)
+            built = ((f * fac[1]).astype(f32) * w).astype(f32) + b
+            alt = ((fac[1] * w).astype(f32) * f).astype(f32) + b
+            assert np.array_equal(built.view(np.uint32), est.view(np.uint32))
+            differs += int((alt.view(np.uint32) != h["prep_est"][i].view(np.uint32)).sum())
+    assert differs > 0
+
+
 @pytest.mark.parametrize("name", H2_NAMES)
 def test_scan_layouts_agree(oracle_mod, name):
     """The vector-major layout read by this build's kernels holds the same
