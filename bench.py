@@ -285,6 +285,8 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: build, 3 launches at --ef (default 50) with "
                          "the tuned launch settings, then a 1 GiB device copy (counter scale)")
+    ap.add_argument("--probe-mode", choices=("sq8", "fastscan"), default="sq8",
+                    help="search path of the --pmc-probe launches (fastscan: default ef 200)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the query path is data-parallel over queries and "
@@ -307,8 +309,13 @@ def main():
     build_s = time.perf_counter() - t
     log(f"build {build_s:.1f}s")
     if args.pmc_probe:
-        ef = args.ef or 50
         db = DeviceBatch(ix, Q, k)
+        if args.probe_mode == "fastscan":
+            ef = args.ef or 200
+            ix.build_sidecar(seed=args.seed)
+            ix.set_mode("fastscan")
+        else:
+            ef = args.ef or 50
         ix.autotune(Q, ef, k)
         for _ in range(3):
             db.launch(ef)

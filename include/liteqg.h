@@ -196,8 +196,10 @@ lqg_status lqg_serve_replay(lqg_server* server, const float* queries, uint64_t n
  * code has no launch geometry). */
 typedef struct {
     int32_t waves_per_group; /* wavefronts (queries in flight) per workgroup, 1..4      */
-    int32_t cache_bits;      /* log2 entries of the per-wave visited cache, 4..14       */
-    int32_t occupancy;       /* 8: kernel capped at 64 VGPRs (8 waves/SIMD); 0: uncapped */
+    int32_t cache_bits;      /* log2 entries of the per-wave visited cache, 4..14 (SQ8
+                                mode); of the LDS visited table, 6..14 (FastScan)    */
+    int32_t occupancy;       /* SQ8 kernel register cap: 8 = 64 VGPRs (8 waves/SIMD),
+                                6 = 80 VGPRs (6 waves/SIMD), 0 = uncapped           */
     int32_t groups;          /* workgroups of the persistent grid; 0 = fill the device  */
 } lqg_launch;
 

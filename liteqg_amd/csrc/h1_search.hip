@@ -178,35 +178,46 @@ __device__ __forceinline__ uint64_t eval_fresh(const DevIndex& ix, const float* 
 // 2^24, so each FP32 operation is exact and the reference's result is exactly
 // sum (q_i - c_i)^2 whatever the order.  That integer is computed here as
 // sum q^2 + sum c^2 - 2 sum q*c with v_dot4_u32_u8 and converted to float
-// (exact below 2^24): bit-identical, with one lane per candidate.
+// (exact below 2^24): bit-identical.  Candidate c's key is returned on lane 2c.
 __device__ __forceinline__ uint64_t eval_fresh_int(const DevIndex& ix, const uint32_t* qp,
                                                    uint32_t qq, const uint32_t* bids, uint32_t m,
                                                    uint32_t lane) {
-    if (lane >= m) return ~0ull;
-    const uint32_t id = bids[lane];
-    const uint4* row = reinterpret_cast<const uint4*>(ix.qdata + static_cast<size_t>(id) * ix.dim);
+    // two lanes per candidate (m <= 32): lane 2c + h takes half h of the row's
+    // 16-byte chunks (at most 8 of the 16 a 256-byte row has); the integer
+    // partial sums are combined with one shuffle
+    const uint32_t c = lane >> 1, h = lane & 1u;
+    const bool act = c < m;
     const uint32_t n16 = ix.dim >> 4;
-    uint32_t qc = 0, cc = 0;
-    for (uint32_t base = 0; base < n16; base += 8) {
+    const uint32_t per = (n16 + 1) >> 1;
+    const uint32_t c0 = h * per;
+    const uint32_t nch = h == 0 ? per : n16 - per;
+    uint32_t qc = 0, cc = 0, id = 0;
+    if (act) {
+        id = bids[c];
+        const uint4* row =
+            reinterpret_cast<const uint4*>(ix.qdata + static_cast<size_t>(id) * ix.dim) + c0;
         uint4 w[8];
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
-            if (base + c < n16) w[c] = row[base + c];
+        for (uint32_t k = 0; k < 8; ++k)
+            if (k < nch) w[k] = row[k];
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            if (base + c < n16) {
-                const uint4 qw = reinterpret_cast<const uint4*>(qp)[base + c];
-                qc = __builtin_amdgcn_udot4(qw.x, w[c].x, qc, false);
-                qc = __builtin_amdgcn_udot4(qw.y, w[c].y, qc, false);
-                qc = __builtin_amdgcn_udot4(qw.z, w[c].z, qc, false);
-                qc = __builtin_amdgcn_udot4(qw.w, w[c].w, qc, false);
-                cc = __builtin_amdgcn_udot4(w[c].x, w[c].x, cc, false);
-                cc = __builtin_amdgcn_udot4(w[c].y, w[c].y, cc, false);
-                cc = __builtin_amdgcn_udot4(w[c].z, w[c].z, cc, false);
-                cc = __builtin_amdgcn_udot4(w[c].w, w[c].w, cc, false);
+        for (uint32_t k = 0; k < 8; ++k) {
+            if (k < nch) {
+                const uint4 qw = reinterpret_cast<const uint4*>(qp)[c0 + k];
+                qc = __builtin_amdgcn_udot4(qw.x, w[k].x, qc, false);
+                qc = __builtin_amdgcn_udot4(qw.y, w[k].y, qc, false);
+                qc = __builtin_amdgcn_udot4(qw.z, w[k].z, qc, false);
+                qc = __builtin_amdgcn_udot4(qw.w, w[k].w, qc, false);
+                cc = __builtin_amdgcn_udot4(w[k].x, w[k].x, cc, false);
+                cc = __builtin_amdgcn_udot4(w[k].y, w[k].y, cc, false);
+                cc = __builtin_amdgcn_udot4(w[k].z, w[k].z, cc, false);
+                cc = __builtin_amdgcn_udot4(w[k].w, w[k].w, cc, false);
             }
         }
     }
+    qc += __shfl_xor(qc, 1);
+    cc += __shfl_xor(cc, 1);
+    if (!act || h) return ~0ull;
     return make_key(static_cast<float>(qq + cc - 2u * qc), id);
 }
 
@@ -222,7 +233,16 @@ __device__ __forceinline__ bool first_visit(uint32_t* bits, uint32_t x) {
     return (atomicOr(&bits[x >> 5], 1u << (x & 31u)) & (1u << (x & 31u))) == 0;
 }
 
-template <int EF_CAP>
+// Evaluation modes of the kernel, fixed at compile time so each variant only
+// holds the registers of its own distance code:
+//   EVAL_INT   -- exact integer path only (eval_fresh_int); queries that do not
+//                 qualify are left for a following EVAL_FLOAT launch
+//   EVAL_FLOAT -- SQ8 float path; with job.skip_int, queries the EVAL_INT
+//                 launch handled are skipped
+constexpr int EVAL_FLOAT = 0;
+constexpr int EVAL_INT = 1;
+
+template <int EF_CAP, int EVAL>
 __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
                            const float* pmin, const float* pscale, const WaveLds& w,
                            uint32_t cache_bits, uint32_t lane, uint32_t* vbits) {
@@ -239,8 +259,15 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
         qint = qint && v >= 0.f && v <= 255.f && v == rintf(v);
     }
     // exact integer path (see eval_fresh_int): identity SQ8, integer query
-    const bool int_path = ix.sq8_identity && (ix.dim & 15u) == 0 && ix.dim <= 256 &&
-                          __ballot(!qint) == 0;
+    const bool int_ok = ix.sq8_identity && (ix.dim & 15u) == 0 && ix.dim <= 256 &&
+                        __ballot(!qint) == 0;
+    if (EVAL == EVAL_INT ? !int_ok : (job.skip_int && int_ok)) {
+        // the other launch of the pair handles this query
+        if (EVAL == EVAL_INT && job.left && lane == 0) atomicAdd(job.left, 1u);
+        wave_sync();
+        return;
+    }
+    constexpr bool int_path = EVAL == EVAL_INT;
     uint32_t qq = 0;
     const uint32_t* qp = reinterpret_cast<const uint32_t*>(w.qv);
     if (int_path) {
@@ -276,11 +303,14 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
     uint64_t P[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) P[s] = ~0ull;
+    auto evaluate = [&](uint32_t m) -> uint64_t {
+        if constexpr (EVAL == EVAL_INT)
+            return eval_fresh_int(ix, qp, qq, w.bids, m, lane);
+        else
+            return eval_fresh(ix, w.qv, pmin, pscale, w.bids, m, lane);
+    };
     {
-        const uint64_t k0 =
-            readlane64(int_path ? eval_fresh_int(ix, qp, qq, w.bids, 1, lane)
-                                : eval_fresh(ix, w.qv, pmin, pscale, w.bids, 1, lane),
-                       0);
+        const uint64_t k0 = readlane64(evaluate(1), 0);
         if (lane == 0) P[0] = k0;
     }
     uint32_t size = 1, n_exp = 0, n_eval = 1;
@@ -328,8 +358,7 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
             } else {
                 n_eval += m;
             }
-            const uint64_t key = int_path ? eval_fresh_int(ix, qp, qq, w.bids, m, lane)
-                                          : eval_fresh(ix, w.qv, pmin, pscale, w.bids, m, lane);
+            const uint64_t key = evaluate(m);
 
             // admission (qg.hpp:329-335): pool not full, or below its maximum
             uint64_t last = 0;
@@ -421,9 +450,9 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
     wave_sync();
 }
 
-// Body shared by the two kernel variants: SQ8 params to LDS, then a
-// persistent loop pulling query indices from the work counter.
-template <int EF_CAP>
+// Body shared by the kernel variants: SQ8 params to LDS, then a persistent
+// loop pulling query indices from the work counter.
+template <int EF_CAP, int EVAL>
 __device__ __forceinline__ void search_kernel_body(const DevIndex& ix, const SearchJob& job,
                                                    uint32_t cache_bits, uint32_t dim_pad,
                                                    uint32_t wave_bytes) {
@@ -445,54 +474,83 @@ __device__ __forceinline__ void search_kernel_body(const DevIndex& ix, const Sea
     w.bids = reinterpret_cast<uint32_t*>(w.stage + EF_CAP);
     w.cache = w.bids + kWave;
 
-    uint32_t* vbits = job.visit_bits
-                          ? job.visit_bits + static_cast<size_t>(blockIdx.x * (blockDim.x / kWave) + wid) *
-                                                 job.visit_words
-                          : nullptr;
+    // a float launch after an integer launch that left nothing: done
+    if (EVAL == EVAL_FLOAT && job.skip_int && job.left && *job.left == 0u) return;
+    // Work distribution: wave gw first takes query gw (no atomic, so the
+    // launch does not start with one atomic per wave on a single counter),
+    // then queries sw + atomicAdd(counter) until the batch is drained, where
+    // sw is the number of waves that take part.  In counting launches a wave
+    // without a bitmap takes no queries.
+    const uint32_t gw = blockIdx.x * (blockDim.x / kWave) + wid;
+    uint32_t sw = gridDim.x * (blockDim.x / kWave);
+    if (job.visit_bits) sw = min(sw, job.visit_waves);
+    if (gw >= sw) return;
+    uint32_t* vbits =
+        job.visit_bits ? job.visit_bits + static_cast<size_t>(gw) * job.visit_words : nullptr;
+    uint32_t q = gw;
     for (;;) {
-        uint32_t q = 0;
-        if (lane == 0) q = atomicAdd(job.counter, 1u);
-        q = __shfl(q, 0);
         if (q >= job.nq) break;
-        search_one<EF_CAP>(ix, job, q, pmin, pscale, w, cache_bits, lane, vbits);
+        search_one<EF_CAP, EVAL>(ix, job, q, pmin, pscale, w, cache_bits, lane, vbits);
+        uint32_t nx = 0;
+        if (lane == 0) nx = sw + atomicAdd(job.counter, 1u);
+        q = __shfl(nx, 0);
     }
 }
 
 // Register allocation left to the compiler.
-template <int EF_CAP>
+template <int EF_CAP, int EVAL>
 __global__ void __launch_bounds__(256) h1_search_kernel(DevIndex ix, SearchJob job,
                                                          uint32_t cache_bits, uint32_t dim_pad,
                                                          uint32_t wave_bytes) {
-    search_kernel_body<EF_CAP>(ix, job, cache_bits, dim_pad, wave_bytes);
+    search_kernel_body<EF_CAP, EVAL>(ix, job, cache_bits, dim_pad, wave_bytes);
 }
 
 // Register file capped at 64 VGPRs so 8 waves fit per SIMD (32 per CU): more
-// queries in flight to hide memory latency, at the price of a few spills.
-template <int EF_CAP>
+// queries in flight to hide memory latency, at the price of spills where the
+// code needs more.
+template <int EF_CAP, int EVAL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
 h1_search_kernel_w8(DevIndex ix, SearchJob job, uint32_t cache_bits, uint32_t dim_pad,
                     uint32_t wave_bytes) {
-    search_kernel_body<EF_CAP>(ix, job, cache_bits, dim_pad, wave_bytes);
+    search_kernel_body<EF_CAP, EVAL>(ix, job, cache_bits, dim_pad, wave_bytes);
 }
 
-__global__ void reset_counter_kernel(uint32_t* counter) { *counter = 0u; }
+// Register file capped at 80 VGPRs: 6 waves per SIMD.
+template <int EF_CAP, int EVAL>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6)))
+h1_search_kernel_w6(DevIndex ix, SearchJob job, uint32_t cache_bits, uint32_t dim_pad,
+                    uint32_t wave_bytes) {
+    search_kernel_body<EF_CAP, EVAL>(ix, job, cache_bits, dim_pad, wave_bytes);
+}
+
+__global__ void reset_counter_kernel(uint32_t* counter, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) counter[i] = 0u;
+}
 
 using KernelFn = void (*)(DevIndex, SearchJob, uint32_t, uint32_t, uint32_t);
 
-template <int EF_CAP>
+template <int EF_CAP, int EVAL>
 KernelFn kernel_for(int occ) {
-    return occ == 8 ? &h1_search_kernel_w8<EF_CAP> : &h1_search_kernel<EF_CAP>;
+    if (occ == 8) return &h1_search_kernel_w8<EF_CAP, EVAL>;
+    if (occ == 6) return &h1_search_kernel_w6<EF_CAP, EVAL>;
+    return &h1_search_kernel<EF_CAP, EVAL>;
 }
 
-KernelFn select_kernel(int ef_cap, int occ) {
+template <int EVAL>
+KernelFn select_for_eval(int ef_cap, int occ) {
     switch (ef_cap) {
-        case 64: return kernel_for<64>(occ);
-        case 128: return kernel_for<128>(occ);
-        case 256: return kernel_for<256>(occ);
-        case 512: return kernel_for<512>(occ);
-        case 1024: return kernel_for<1024>(occ);
+        case 64: return kernel_for<64, EVAL>(occ);
+        case 128: return kernel_for<128, EVAL>(occ);
+        case 256: return kernel_for<256, EVAL>(occ);
+        case 512: return kernel_for<512, EVAL>(occ);
+        case 1024: return kernel_for<1024, EVAL>(occ);
         default: return nullptr;
     }
+}
+
+KernelFn select_kernel(const LaunchCfg& cfg) {
+    return cfg.eval == EVAL_INT ? select_for_eval<EVAL_INT>(cfg.ef_cap, cfg.occupancy)
+                                : select_for_eval<EVAL_FLOAT>(cfg.ef_cap, cfg.occupancy);
 }
 
 void geometry(const DevIndex& ix, const LaunchCfg& cfg, uint32_t& dim_pad, uint32_t& wave_bytes) {
@@ -516,7 +574,7 @@ int search_full_grid(const DevIndex& ix, const LaunchCfg& cfg) {
     if (hipGetDevice(&dev) != hipSuccess) return 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return 1;
-    KernelFn k = select_kernel(cfg.ef_cap, cfg.occupancy);
+    KernelFn k = select_kernel(cfg);
     if (!k) return 1;
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k),
@@ -536,7 +594,7 @@ int search_groups(const DevIndex& ix, const SearchJob& job, const LaunchCfg& cfg
 
 hipError_t launch_search(const DevIndex& ix, const SearchJob& job, const LaunchCfg& cfg,
                          hipStream_t stream) {
-    KernelFn k = select_kernel(cfg.ef_cap, cfg.occupancy);
+    KernelFn k = select_kernel(cfg);
     if (!k || cfg.cache_bits < 4 || cfg.cache_bits > 14 || cfg.waves_per_group < 1 ||
         cfg.waves_per_group > 4)
         return hipErrorInvalidValue;
@@ -555,8 +613,8 @@ hipError_t launch_search(const DevIndex& ix, const SearchJob& job, const LaunchC
     return hipGetLastError();
 }
 
-hipError_t launch_reset_counter(uint32_t* counter, hipStream_t stream) {
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, stream, counter);
+hipError_t launch_reset_counter(uint32_t* counter, hipStream_t stream, uint32_t n) {
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, counter, n);
     return hipGetLastError();
 }
 

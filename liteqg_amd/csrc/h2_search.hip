@@ -616,12 +616,17 @@ __global__ void __launch_bounds__(256) h2_search_kernel(H2Dev ix, H2Job job, uin
     const H2Lds w = carve(lds + static_cast<size_t>(wid) * lay.wave_bytes, lay);
     const uint32_t gwave = blockIdx.x * (blockDim.x / kWave) + wid;
     uint32_t* gtab = job.gvis ? job.gvis + (static_cast<size_t>(gwave) << job.gvis_bits) : nullptr;
+    // wave gwave first takes query gwave, then sw + atomicAdd(counter) (sw =
+    // waves in the grid), so the launch does not start with one atomic per
+    // wave on a single counter
+    const uint32_t sw = gridDim.x * (blockDim.x / kWave);
+    uint32_t q = gwave;
     for (;;) {
-        uint32_t q = 0;
-        if (lane == 0) q = atomicAdd(job.counter, 1u);
-        q = __shfl(q, 0);
         if (q >= job.nq) break;
         h2_search_one<EF_CAP, CW>(ix, job, q, w, hash_bits, gtab, lane);
+        uint32_t nx = 0;
+        if (lane == 0) nx = sw + atomicAdd(job.counter, 1u);
+        q = __shfl(nx, 0);
     }
 }
 

@@ -135,7 +135,8 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
         return run_h2_batch(h, d_queries, nq, d_ef, d_k, ef_max, kmax, d_results, d_dists, d_stats,
                             s, sc);
     }
-    LQG_HIP_TRY(sc->counter.reserve(1));
+    if (nq > 0x7FFFFFFFull) return fail(LQG_ERR_ARG, "nq must be below 2^31");
+    LQG_HIP_TRY(sc->counter.reserve(3));
     if (ef_class(ef_max) == 0)
         return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported on the device");
     SearchJob job;
@@ -153,19 +154,43 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
     job.counter = sc->counter.p;
     job.visit_bits = nullptr;
     job.visit_words = 0;
+    job.visit_waves = 0;
     const int cls = ef_class(ef_max);
     LaunchCfg cfg = h->launch_set[class_slot(cls)] ? h->launch[class_slot(cls)] : default_launch(cls);
     cfg.ef_cap = cls;
     if (nq < static_cast<uint64_t>(cfg.waves_per_group)) cfg.waves_per_group = 1;
+    // An index whose SQ8 decode is exact (min 0, scale 1; D % 16 == 0, D <=
+    // 256) runs the integer-path kernel first; a float-path launch then takes
+    // only the queries that were not integer valued.  Each kernel variant
+    // holds only its own distance code's registers.
+    const bool int_index = h->sq8_identity && h->dim % 16 == 0 && h->dim <= 256;
     if (count_visits) {
-        // one N-bit bitmap per wave of the grid
-        const int groups = search_groups(h->dev(), job, cfg);
+        // one N-bit bitmap per wave, for the larger grid of the two kernel
+        // variants (their occupancy, hence their grid, differs)
+        LaunchCfg ci = cfg, cf = cfg;
+        ci.eval = 1;
+        cf.eval = 0;
+        int groups = search_groups(h->dev(), job, cf);
+        if (int_index) groups = std::max(groups, search_groups(h->dev(), job, ci));
         job.visit_words = static_cast<uint32_t>((h->n + 31) / 32);
-        const size_t words = static_cast<size_t>(groups) * cfg.waves_per_group * job.visit_words;
+        job.visit_waves = static_cast<uint32_t>(groups) * static_cast<uint32_t>(cfg.waves_per_group);
+        const size_t words = static_cast<size_t>(job.visit_waves) * job.visit_words;
         LQG_HIP_TRY(h->d_visit.reserve(words));
         job.visit_bits = h->d_visit.p;
     }
-    LQG_HIP_TRY(launch_reset_counter(sc->counter.p, s));
+    // counters: [0] integer launch queue, [1] float launch queue, [2] queries
+    // the integer launch left to the float launch
+    LQG_HIP_TRY(launch_reset_counter(sc->counter.p, s, 3));
+    if (int_index) {
+        cfg.eval = 1;
+        job.skip_int = 0;
+        job.left = sc->counter.p + 2;
+        LQG_HIP_TRY(launch_search(h->dev(), job, cfg, s));
+    }
+    cfg.eval = 0;
+    job.skip_int = int_index ? 1 : 0;
+    job.left = int_index ? sc->counter.p + 2 : nullptr;
+    job.counter = sc->counter.p + 1;
     LQG_HIP_TRY(launch_search(h->dev(), job, cfg, s));
     return LQG_OK;
 }
@@ -366,7 +391,7 @@ lqg_status lqg_search_batch(lqg_index* h, const float* queries, uint64_t nq, con
                             const uint32_t* k, uint32_t kmax, lqg_order order, uint32_t* results,
                             float* dists, lqg_stats* stats) {
     if (!h || !queries || !results) return fail(LQG_ERR_ARG, "null argument");
-    if (nq == 0 || nq > 0xFFFFFFFFull) return fail(LQG_ERR_ARG, "nq must be in [1, 2^32)");
+    if (nq == 0 || nq > 0x7FFFFFFFull) return fail(LQG_ERR_ARG, "nq must be in [1, 2^31)");
     if (kmax == 0) return fail(LQG_ERR_ARG, "kmax must be positive");
     if (!valid_order(order)) return fail(LQG_ERR_ARG, "unknown order");
     if (!ef && !h->ef_set) return fail(LQG_ERR_STATE, "set_ef() or per-query ef required");
@@ -433,7 +458,7 @@ lqg_status lqg_search_batch_device(lqg_index* h, const float* d_queries, uint64_
                                    uint32_t kmax, lqg_order order, uint32_t* d_results,
                                    float* d_dists, uint32_t* d_stats, void* stream) {
     if (!h || !d_queries || !d_results) return fail(LQG_ERR_ARG, "null argument");
-    if (nq == 0 || nq > 0xFFFFFFFFull) return fail(LQG_ERR_ARG, "nq must be in [1, 2^32)");
+    if (nq == 0 || nq > 0x7FFFFFFFull) return fail(LQG_ERR_ARG, "nq must be in [1, 2^31)");
     if (kmax == 0) return fail(LQG_ERR_ARG, "kmax must be positive");
     if (!valid_order(order)) return fail(LQG_ERR_ARG, "unknown order");
     if (ef_max == 0) {
@@ -453,7 +478,8 @@ lqg_status lqg_set_launch(lqg_index* h, uint32_t ef, const lqg_launch* cfg) {
     if (!h) return fail(LQG_ERR_ARG, "null index");
     if (ef != 0 && ef_class(ef) == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024");
     if (cfg && (cfg->waves_per_group < 1 || cfg->waves_per_group > 4 || cfg->cache_bits < 4 ||
-                cfg->cache_bits > 14 || (cfg->occupancy != 0 && cfg->occupancy != 8) ||
+                cfg->cache_bits > 14 ||
+                (cfg->occupancy != 0 && cfg->occupancy != 6 && cfg->occupancy != 8) ||
                 cfg->groups < 0))
         return fail(LQG_ERR_ARG, "launch settings out of range");
     const bool fs = h->mode == LQG_MODE_FASTSCAN;
@@ -584,7 +610,7 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
     const lqg::LaunchCfg saved = h->launch[slot];
     float best_ms = 1e30f;
     lqg::LaunchCfg best_cfg = lqg::default_launch(cls);
-    const int occs[] = {0, 8};
+    const int occs[] = {0, 6, 8};
     const int bits[] = {8, 9, 10, 11};
     const int wpg[] = {1, 2, 4};
     for (int occ : occs)

@@ -21,7 +21,7 @@ lqg_status lqg_graph_create(lqg_index* h, const float* d_queries, uint64_t nq,
                             uint32_t* d_stats, lqg_batch_graph** out) {
     if (!h || !d_queries || !d_results || !out) return fail(LQG_ERR_ARG, "null argument");
     *out = nullptr;
-    if (nq == 0 || nq > 0xFFFFFFFFull) return fail(LQG_ERR_ARG, "nq must be in [1, 2^32)");
+    if (nq == 0 || nq > 0x7FFFFFFFull) return fail(LQG_ERR_ARG, "nq must be in [1, 2^31)");
     if (kmax == 0) return fail(LQG_ERR_ARG, "kmax must be positive");
     if (order != LQG_ORDER_REFERENCE && order != LQG_ORDER_NEAREST)
         return fail(LQG_ERR_ARG, "unknown order");

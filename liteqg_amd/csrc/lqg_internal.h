@@ -36,6 +36,13 @@ struct SearchJob {
     // (E, number of distinct nodes evaluated) instead of (E, evaluations).
     uint32_t* visit_bits;
     uint32_t visit_words;
+    uint32_t visit_waves = 0;  // bitmaps allocated; a wave beyond them counts nothing
+    // Float-path launch that follows an integer-path launch over the same
+    // queries: skip the queries the integer path took.
+    int skip_int = 0;
+    // Queries the integer-path launch left to the float path (counted by the
+    // integer launch, read by the float launch, which returns when it is 0).
+    uint32_t* left = nullptr;
 };
 
 // Launch geometry chosen by the host (autotuner or defaults).
@@ -44,7 +51,8 @@ struct LaunchCfg {
     int waves_per_group; // waves per workgroup
     int cache_bits;      // log2 of the per-wave visited cache (4..14)
     int groups;          // workgroups in the persistent grid (0 = fill the device)
-    int occupancy;       // 8: kernel variant capped at 64 VGPRs (8 waves/SIMD); 0: uncapped
+    int occupancy;       // register cap: 8 = 64 VGPRs, 6 = 80 VGPRs, 0 = uncapped
+    int eval = 0;        // 0: SQ8 float distances; 1: exact integer distances (v_dot4)
 };
 
 // Device view of an index for the FastScan RaBitQ search (search_qg,
@@ -110,7 +118,7 @@ hipError_t launch_search(const DevIndex& ix, const SearchJob& job, const LaunchC
                          hipStream_t stream);
 
 // Zeroes the work-queue counter with a kernel (capturable in a HIP graph).
-hipError_t launch_reset_counter(uint32_t* counter, hipStream_t stream);
+hipError_t launch_reset_counter(uint32_t* counter, hipStream_t stream, uint32_t n = 1);
 
 // Number of workgroups that fill the device for this configuration.
 int search_full_grid(const DevIndex& ix, const LaunchCfg& cfg);

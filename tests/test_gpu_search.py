@@ -226,6 +226,26 @@ def test_count_visits_equals_reference_visited_set(oracle_mod, name):
             assert (got[0, 0], got[0, 1]) == (E, V)
 
 
+def test_integer_index_full_device_batch(oracle_mod, tmp_path):
+    """An integer index (exact SQ8 decode) with a batch large enough to fill
+    the device: integer-valued queries take the integer kernel, the others the
+    float kernel that follows it; results, and the exact visit counts (one
+    bitmap per wave of the larger of the two grids), equal the oracle's."""
+    arr, base = make_random_index(3000, 64, 32, seed=17, kind="int", knn=True)
+    assert np.all(arr.params[:, 0] == 0) and np.all(arr.params[:, 1] == 1)
+    ix = device_from_arrays(arr, tmp_path)
+    rng = np.random.default_rng(4)
+    nq = 12000
+    Q = np.clip(base[rng.integers(0, 3000, nq)] + rng.integers(-3, 4, (nq, 64)), 0, 255)
+    Q = Q.astype(np.float32)
+    Q[::7] += 0.25  # every 7th query is not integer valued
+    ids, dis = ix.search_batch(Q, 10, ef=50, order="nearest", return_distances=True)
+    exp, exp_d, est = oracle_batch(oracle_mod, arr, Q, 50, 10, "nearest")
+    assert_same(ids, dis, exp, exp_d)
+    got = ix.count_visits(Q, 50, 10).astype(np.int64)
+    assert np.array_equal(got, est.astype(np.int64))
+
+
 def test_count_visits_with_tiny_cache(oracle_mod, tmp_path):
     """With a 16-entry cache the kernel re-evaluates many nodes; the exact count
     still equals the oracle's and is below the evaluation count."""
