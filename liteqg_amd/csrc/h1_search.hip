@@ -311,7 +311,11 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
     };
     {
         const uint64_t k0 = readlane64(evaluate(1), 0);
-        if (lane == 0) P[0] = k0;
+        if (lane == 0) {
+            P[0] = k0;
+            w.stage[0] = k0;  // the stage array mirrors the pool between merges
+        }
+        wave_sync();
     }
     uint32_t size = 1, n_exp = 0, n_eval = 1;
 
@@ -369,37 +373,56 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
             bool cand = key != ~0ull && (size < ef || key < last);
             uint64_t mm = __ballot(cand);
             if (mm == 0) continue;
-            // one pass over the candidates: position in the pool, duplicate test,
-            // rank among candidates, shift of every pool entry.  Candidate keys
-            // have bit 0 clear, so plain comparisons with pool keys order by
-            // (distance, id) except for a candidate equal to a pool entry, which
-            // is detected first and dropped.
+            // One pass over the candidates (broadcast from their lanes): each
+            // candidate's position below the pool (lb, written into its lane),
+            // its rank among the candidates, and every pool entry's shift (the
+            // candidates below it).  Candidate keys have bit 0 clear, so these
+            // plain comparisons order by (distance, id) as long as no candidate is
+            // already in the pool -- a node evaluated before whose cache entry
+            // was overwritten.  Such a candidate's pool copy is the entry at its
+            // position lb; the stage array holds the pool as of the last merge
+            // (only expanded bits may differ), so one LDS read per candidate
+            // finds them, and only then is the pass redone with duplicates
+            // dropped.
             uint32_t lb = 0, rank = 0;
             uint32_t shift[S];
 #pragma unroll
             for (int s = 0; s < S; ++s) shift[s] = 0;
-            bool drop = false;
-            while (mm) {
-                const int j = static_cast<int>(__builtin_ctzll(mm));
-                mm &= mm - 1;
-                const uint64_t kj = readlane64(key, j);
-                uint32_t below = 0;
-                bool eq = false;
+            {
+                uint64_t rest = mm;
+                while (rest) {
+                    const int j = static_cast<int>(__builtin_ctzll(rest));
+                    rest &= rest - 1;
+                    const uint64_t kj = readlane64(key, j);
+                    uint32_t below = 0;
 #pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    below += static_cast<uint32_t>(__popcll(__ballot(P[s] < kj)));
-                    eq |= __ballot(((P[s] ^ kj) >> 1) == 0) != 0;
+                    for (int s = 0; s < S; ++s) {
+                        const bool lt = P[s] < kj;
+                        below += static_cast<uint32_t>(__popcll(__ballot(lt)));
+                        shift[s] += lt ? 0u : 1u;
+                    }
+                    rank += kj < key ? 1u : 0u;
+                    if (static_cast<int>(lane) == j) lb = below;
                 }
-                if (eq) {  // already in the pool (evaluated before, forgotten by the cache)
-                    if (static_cast<int>(lane) == j) drop = true;
-                    continue;
-                }
-                if (static_cast<int>(lane) == j) lb = below;
-                rank += kj < key ? 1u : 0u;
-#pragma unroll
-                for (int s = 0; s < S; ++s) shift[s] += kj < P[s] ? 1u : 0u;
             }
-            cand = cand && !drop;
+            bool drop = cand && lb < size && ((w.stage[lb] ^ key) >> 1) == 0;
+            if (__ballot(drop)) {
+                // slow pass with the duplicates dropped
+                cand = cand && !drop;
+                mm = __ballot(cand);
+                rank = 0;
+#pragma unroll
+                for (int s = 0; s < S; ++s) shift[s] = 0;
+                uint64_t rest = mm;
+                while (rest) {
+                    const int j = static_cast<int>(__builtin_ctzll(rest));
+                    rest &= rest - 1;
+                    const uint64_t kj = readlane64(key, j);
+                    rank += kj < key ? 1u : 0u;
+#pragma unroll
+                    for (int s = 0; s < S; ++s) shift[s] += kj < P[s] ? 1u : 0u;
+                }
+            }
             const uint32_t m3 = static_cast<uint32_t>(__popcll(__ballot(cand)));
             if (m3 == 0) continue;
             // merge: scatter to new positions, keep ef (res_pool.pop, qg.hpp:332-333)
