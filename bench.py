@@ -283,10 +283,15 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--plain", type=int, default=500, help="queries timed on the plain path")
     ap.add_argument("--pmc-probe", action="store_true",
-                    help="for rocprofv3 --pmc runs: build, 3 launches at --ef (default 50) with "
+                    help="for rocprofv3 runs: build, --probe-launches launches at --ef (default 50) with "
                          "the tuned launch settings, then a 1 GiB device copy (counter scale)")
     ap.add_argument("--probe-mode", choices=("sq8", "fastscan"), default="sq8",
                     help="search path of the --pmc-probe launches (fastscan: default ef 200)")
+    ap.add_argument("--probe-launches", type=int, default=3,
+                    help="number of --pmc-probe search launches")
+    ap.add_argument("--launch", default="",
+                    help="pin the launch settings of the probe instead of autotuning: "
+                         "waves_per_group,cache_bits,occupancy[,groups]")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the query path is data-parallel over queries and "
@@ -316,15 +321,18 @@ def main():
             ix.set_mode("fastscan")
         else:
             ef = args.ef or 50
-        ix.autotune(Q, ef, k)
-        for _ in range(3):
-            db.launch(ef)
-        torch.cuda.synchronize()
+        if args.launch:
+            v = [int(x) for x in args.launch.split(",")]
+            ix.set_launch(ef, *v)
+        else:
+            ix.autotune(Q, ef, k)
+        ms, _ = time_launches(db, ef, args.probe_launches, 0)
         x = torch.zeros(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
         y.copy_(x)  # reads 1 GiB, writes 1 GiB
         torch.cuda.synchronize()
-        log(f"pmc probe: 3 launches at ef={ef}, launch {ix.get_launch(ef)}, then a 1 GiB copy")
+        log(f"pmc probe: {args.probe_launches} launches at ef={ef}, launch {ix.get_launch(ef)}, "
+            f"{ms:.4f} ms per launch (HIP events), then a 1 GiB copy")
         return
     t = time.perf_counter()
     gt = ground_truth(base, Q, k)

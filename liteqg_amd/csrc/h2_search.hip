@@ -62,11 +62,11 @@ __device__ __forceinline__ uint64_t sb_key(float est, uint32_t seq) {
 
 struct H2Lds {
     float* qv;        // query (dim)
-    float* tb;        // rotated query during preparation, then q - x (p floats)
+    float* tb;        // rotated query during preparation, q - x in exact_l2 (p floats)
     uint8_t* qb;      // quantised query (p bytes)
     uint8_t* lut;     // 4p bytes (+16 bytes of bank skew for 2 lanes per vector)
-    uint64_t* sk;     // pool merge staging: keys (EF_CAP)
-    uint32_t* si;     // pool merge staging: ids (EF_CAP)
+    uint64_t* sk;     // pool merge staging: keys (EF_CAP + 1; shares tb's memory)
+    uint32_t* si;     // pool merge staging: ids (EF_CAP + 1)
     uint32_t* hv;     // visited table (1 << hash_bits)
     float* rd;        // ResultBuffer distances (kcap)
     uint32_t* ri;     // ResultBuffer ids (kcap)
@@ -85,11 +85,16 @@ __host__ __device__ inline Layout h2_layout(uint32_t dim, uint32_t p, uint32_t e
     const uint32_t kcap = kmax + 1;
     uint32_t o = 0;
     l.qv = o;   o += align16(4 * dim);
-    l.tb = o;   o += align16(4 * p);
-    l.qb = o;   o += align16(p);
     l.lut = o;  o += align16(4 * p + 16);
-    l.sk = o;   o += 8 * ef_cap;
-    l.si = o;   o += align16(4 * ef_cap);
+    // one region for the preparation buffers (tb, qb) and, once the LUT is
+    // built, the pool merge staging (sk, si: ef_cap entries and a spare)
+    const uint32_t prep = align16(4 * p) + align16(p);
+    const uint32_t stage = align16(8 * (ef_cap + 1)) + align16(4 * (ef_cap + 1));
+    l.tb = o;
+    l.qb = o + align16(4 * p);
+    l.sk = o;
+    l.si = o + align16(8 * (ef_cap + 1));
+    o += prep > stage ? prep : stage;
     l.hv = o;   o += 4u << hash_bits;
     l.rd = o;   o += align16(4 * kcap);
     l.ri = o;   o += align16(4 * kcap);
@@ -219,6 +224,54 @@ __device__ float exact_l2(const float* row, const H2Lds& w, uint32_t dim, uint32
     return r;
 }
 
+// The same distance with the partials in registers: lane p < 16 runs the FMA
+// chain over dims p, p + 16, ... (the first kRegJ of them from xr / qr, which
+// the caller loaded, the rest from the row and w.qv), the tree is taken with
+// DPP row shifts (lane p adds lane p + 8, p + 4, p + 2, p + 1 of its row:
+// the pairs of reduce_l2), and lane 0's sum is broadcast.
+constexpr int kRegJ = 8;
+
+__device__ __forceinline__ float dpp_shl(float x, int ctrl_n) {
+    // row_shl:n, lanes past the row end read 0
+    int r = 0;
+    switch (ctrl_n) {
+        case 8: r = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x108, 0xF, 0xF, false); break;
+        case 4: r = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x104, 0xF, 0xF, false); break;
+        case 2: r = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x102, 0xF, 0xF, false); break;
+        default: r = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x101, 0xF, 0xF, false); break;
+    }
+    return __int_as_float(r);
+}
+
+__device__ __forceinline__ float exact_l2_regs(const float (&xr)[kRegJ], const float (&qr)[kRegJ],
+                                               const float* row, const H2Lds& w, uint32_t dim,
+                                               uint32_t lane) {
+    const uint32_t nj = dim / 16;  // chain length of every partial
+    float s = 0.0f;
+#pragma unroll
+    for (int u = 0; u < kRegJ; ++u) {
+        if (static_cast<uint32_t>(u) < nj) {
+            const float t = qr[u] - xr[u];
+            s = fmaf(t, t, s);
+        }
+    }
+    if (nj > static_cast<uint32_t>(kRegJ) && lane < 16)
+        for (uint32_t j = kRegJ; j < nj; ++j) {
+            const float t = w.qv[lane + 16 * j] - row[lane + 16 * j];
+            s = fmaf(t, t, s);
+        }
+    s = s + dpp_shl(s, 8);
+    s = s + dpp_shl(s, 4);
+    s = s + dpp_shl(s, 2);
+    s = s + dpp_shl(s, 1);
+    float r = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(s)));
+    for (uint32_t i = nj * 16; i < dim; ++i) {
+        const float t = w.qv[i] - row[i];
+        r = fmaf(t, t, r);
+    }
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // Exact visited set.  LDS table while it is at most half full, then a table
 // of (1 << gbits) entries in device memory (atomics, so every lane sees every
@@ -250,6 +303,23 @@ __device__ __forceinline__ bool v_contains(const Visited& v, uint32_t id) {
     uint32_t h = vhash(id, v.gbits);
     for (;;) {
         const uint32_t e = __hip_atomic_load(&v.glb[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (e == id) return true;
+        if (e == kEmpty) return false;
+        h = (h + 1u) & mask;
+    }
+}
+
+// v_contains for an id that is the same on every lane: the probe loop runs on
+// scalar values.
+__device__ __forceinline__ bool v_contains_u(const Visited& v, uint32_t id) {
+    const uint32_t bits = v.global ? v.gbits : v.lbits;
+    const uint32_t mask = (1u << bits) - 1u;
+    uint32_t h = vhash(id, bits);
+    for (;;) {
+        const uint32_t raw = v.global
+                                 ? __hip_atomic_load(&v.glb[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                 : v.lds[h];
+        const uint32_t e = __builtin_amdgcn_readfirstlane(raw);
         if (e == id) return true;
         if (e == kEmpty) return false;
         h = (h + 1u) & mask;
@@ -289,15 +359,13 @@ __device__ void v_insert(Visited& v, uint32_t id, uint32_t lane) {
         v.failed = true;
         return;
     }
-    if (lane == 0) {
-        if (!v.global) {
-            const uint32_t mask = (1u << v.lbits) - 1u;
-            uint32_t h = vhash(id, v.lbits);
-            while (v.lds[h] != kEmpty) h = (h + 1u) & mask;
-            v.lds[h] = id;
-        } else {
-            g_insert(v.glb, v.gbits, id);
-        }
+    if (!v.global) {
+        const uint32_t mask = (1u << v.lbits) - 1u;
+        uint32_t h = vhash(id, v.lbits);
+        while (__builtin_amdgcn_readfirstlane(v.lds[h]) != kEmpty) h = (h + 1u) & mask;
+        if (lane == 0) v.lds[h] = id;
+    } else if (lane == 0) {
+        g_insert(v.glb, v.gbits, id);
     }
     if (v.global) __threadfence();
     wave_sync();
@@ -437,27 +505,40 @@ __device__ void h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
         I[0] = ix.ep;
     }
     uint32_t size = 1, seq = 1, n_exp = 0, n_pop = 0, rsize = 0;
+    float rtail = INFINITY;  // ResultBuffer's last distance once it holds kq entries
+    // query dims lane + 16u (u < kRegJ) for the partials of exact_l2_regs
+    const uint32_t mul16 = dim & ~15u;
+    float qr[kRegJ];
+#pragma unroll
+    for (int u = 0; u < kRegJ; ++u) {
+        const uint32_t d = lane + 16u * static_cast<uint32_t>(u);
+        qr[u] = (lane < 16 && d < mul16) ? w.qv[d] : 0.0f;
+    }
 
     for (;;) {
         // pop: first unchecked entry (buffer.hpp:52-60)
-        uint32_t cur = 0;
-        bool found = false;
+        uint64_t pm = 0;
+        int fs = S;
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-            if (!found) {
-                const uint32_t i = static_cast<uint32_t>(s * kWave) + lane;
-                const uint64_t m = __ballot(i < size && !(P[s] & 1ull));
-                if (m) {
-                    const int l = static_cast<int>(__builtin_ctzll(m));
-                    cur = readlane32(I[s], l);
-                    if (static_cast<int>(lane) == l) P[s] |= 1ull;
-                    found = true;
-                }
+        for (int s = S - 1; s >= 0; --s) {
+            const uint32_t i = static_cast<uint32_t>(s * kWave) + lane;
+            const uint64_t m = __ballot(i < size && !(P[s] & 1ull));
+            if (m) {
+                pm = m;
+                fs = s;
             }
         }
-        if (!found) break;
+        if (fs == S) break;
+        const int pl = static_cast<int>(__builtin_ctzll(pm));
+        uint32_t cur = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const uint32_t c = readlane32(I[s], pl);
+            if (s == fs) cur = c;
+            P[s] |= (s == fs && static_cast<int>(lane) == pl) ? 1ull : 0ull;
+        }
         ++n_pop;
-        if (v_contains(vis, cur)) continue;
+        if (v_contains_u(vis, cur)) continue;
         v_insert(vis, cur, lane);
         if (vis.failed) break;
         ++n_exp;
@@ -467,10 +548,17 @@ __device__ void h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
         const unsigned char* codes = rec + ix.off_codes;
         const float* fac = reinterpret_cast<const float*>(rec + ix.off_fac);
         const uint32_t* ids = reinterpret_cast<const uint32_t*>(rec + ix.off_ids);
+        // the node's raw row for the exact distance: dims lane + 16u
+        float xr[kRegJ];
+#pragma unroll
+        for (int u = 0; u < kRegJ; ++u) {
+            const uint32_t d = lane + 16u * static_cast<uint32_t>(u);
+            xr[u] = (lane < 16 && d < mul16) ? raw[d] : 0.0f;
+        }
         float sqr_y = 0.0f;
 
         for (uint32_t vb = 0; vb < deg; vb += vp) {
-            // the pass's loads first (codes, factors, id; the raw row on pass 0)
+            // the pass's loads first (codes, factors, id)
             const uint32_t vi = vb + lane / lpv;
             const bool act = vi < deg;
             uint32_t cw[CW];
@@ -487,34 +575,21 @@ __device__ void h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
 #pragma unroll
                 for (int k = 0; k < CW; ++k) cw[k] = 0;
             }
-            if (vb == 0) {
-                // scan_neighbors (qg.hpp:355): exact distance to the node
-                if ((dim & 3u) == 0) {
-                    for (uint32_t i = 4 * lane; i < dim; i += 4 * kWave) {
-                        const float4 x = *reinterpret_cast<const float4*>(raw + i);
-                        w.tb[i] = w.qv[i] - x.x;
-                        w.tb[i + 1] = w.qv[i + 1] - x.y;
-                        w.tb[i + 2] = w.qv[i + 2] - x.z;
-                        w.tb[i + 3] = w.qv[i + 3] - x.w;
-                    }
-                } else {
-                    for (uint32_t i = lane; i < dim; i += kWave) w.tb[i] = w.qv[i] - raw[i];
-                }
-                wave_sync();
-                sqr_y = reduce_l2(w.tb, dim, lane);
-                wave_sync();
-            }
+            // scan_neighbors (qg.hpp:355): exact distance to the node
+            if (vb == 0) sqr_y = exact_l2_regs(xr, qr, raw, w, dim, lane);
             const float est = lane_estimate<CW>(cw, lut_part, lpv, pr, sqr_y, tx, dq, vq);
             // is_full / visited_.get (qg.hpp:375-377); candidate i gets insertion
-            // number seq + i
-            bool cand = act && part == 0 && !v_contains(vis, nid);
+            // number seq + i.  The visited table is probed only by the lanes the
+            // pool would take.
             const uint64_t key = sb_key(est, seq + vi);
             uint64_t last = 0;
 #pragma unroll
             for (int s = 0; s < S; ++s)
                 if (static_cast<uint32_t>(s) == (size - 1) / kWave)
                     last = readlane64(P[s], static_cast<int>((size - 1) % kWave));
-            cand = cand && (size < ef || key < last);
+            bool cand = act && part == 0 && (size < ef || key < last);
+            if (__ballot(cand) == 0) continue;
+            if (cand) cand = !v_contains(vis, nid);
             uint64_t mm = __ballot(cand);
             if (mm == 0) continue;
             uint32_t lb = 0, rank = 0;
@@ -535,31 +610,41 @@ __device__ void h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
             }
             const uint32_t m3 = static_cast<uint32_t>(__popcll(__ballot(cand)));
             const uint32_t nsize = min(size + m3, ef);
+            // merge through LDS; entries that fall off the end go to the spare
+            // slot EF_CAP, so the stores need no branch
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 const uint32_t i = static_cast<uint32_t>(s * kWave) + lane;
                 const uint32_t np = i + shift[s];
-                if (i < size && np < nsize) {
-                    w.sk[np] = P[s];
-                    w.si[np] = I[s];
-                }
+                const uint32_t dst = (i < size && np < nsize) ? np : static_cast<uint32_t>(EF_CAP);
+                w.sk[dst] = P[s];
+                w.si[dst] = I[s];
             }
-            if (cand && lb + rank < nsize) {
-                w.sk[lb + rank] = key;
-                w.si[lb + rank] = nid;
+            {
+                const uint32_t np = lb + rank;
+                const uint32_t dst = (cand && np < nsize) ? np : static_cast<uint32_t>(EF_CAP);
+                w.sk[dst] = key;
+                w.si[dst] = nid;
             }
             wave_sync();
 #pragma unroll
             for (int s = 0; s < S; ++s) {
                 const uint32_t i = static_cast<uint32_t>(s * kWave) + lane;
-                P[s] = i < nsize ? w.sk[i] : ~0ull;
-                I[s] = i < nsize ? w.si[i] : 0u;
+                const uint64_t kv = w.sk[i];
+                const uint32_t iv = w.si[i];
+                P[s] = i < nsize ? kv : ~0ull;
+                I[s] = i < nsize ? iv : 0u;
             }
             size = nsize;
             wave_sync();
         }
         seq += deg;
-        rb_insert(w, rsize, kq, cur, sqr_y, lane);  // res_pool.insert (qg.hpp:282)
+        // res_pool.insert (qg.hpp:282); skipped without a look at LDS when the
+        // buffer is full and the distance is above its last
+        if (kq != 0 && !(rsize == kq && sqr_y > rtail)) {
+            rb_insert(w, rsize, kq, cur, sqr_y, lane);
+            if (rsize == kq) rtail = w.rd[kq - 1];
+        }
     }
 
     // update_results (qg.hpp:395-418): fewer than k nodes expanded
