@@ -198,8 +198,10 @@ typedef struct {
     int32_t waves_per_group; /* wavefronts (queries in flight) per workgroup, 1..4      */
     int32_t cache_bits;      /* log2 entries of the per-wave visited cache, 4..14 (SQ8
                                 mode); of the LDS visited table, 6..14 (FastScan)    */
-    int32_t occupancy;       /* SQ8 kernel register cap: 8 = 64 VGPRs (8 waves/SIMD),
-                                6 = 80 VGPRs (6 waves/SIMD), 0 = uncapped           */
+    int32_t occupancy;       /* kernel register cap, 0 = uncapped.  SQ8: 8 = 64 VGPRs
+                                (8 waves/SIMD), 6 = 80 VGPRs (6 waves/SIMD).
+                                FastScan: 5 or 6 waves/SIMD, for indexes with
+                                P / lanes per neighbour <= 128 code bits        */
     int32_t groups;          /* workgroups of the persistent grid; 0 = fill the device  */
 } lqg_launch;
 

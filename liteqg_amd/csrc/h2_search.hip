@@ -309,6 +309,24 @@ __device__ __forceinline__ bool v_contains(const Visited& v, uint32_t id) {
     }
 }
 
+// v_contains on the lanes where `active` is set (false on the others).  The
+// LDS probe runs without divergent branches: every lane reads a slot of the
+// table each round (inactive ones a harmless in-range slot) until every
+// active lane has met its id or an empty slot.
+__device__ __forceinline__ bool v_contains_lanes(const Visited& v, uint32_t id, bool active) {
+    if (v.global) return active && v_contains(v, id);
+    const uint32_t mask = (1u << v.lbits) - 1u;
+    uint32_t h = vhash(id, v.lbits);
+    bool found = false;
+    while (__ballot(active)) {
+        const uint32_t e = v.lds[h];
+        found = found || (active && e == id);
+        active = active && e != id && e != kEmpty;
+        h = (h + 1u) & mask;
+    }
+    return found;
+}
+
 // v_contains for an id that is the same on every lane: the probe loop runs on
 // scalar values.
 __device__ __forceinline__ bool v_contains_u(const Visited& v, uint32_t id) {
@@ -589,27 +607,34 @@ __device__ void h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
                     last = readlane64(P[s], static_cast<int>((size - 1) % kWave));
             bool cand = act && part == 0 && (size < ef || key < last);
             if (__ballot(cand) == 0) continue;
-            if (cand) cand = !v_contains(vis, nid);
+            cand = cand && !v_contains_lanes(vis, nid, cand);
             uint64_t mm = __ballot(cand);
             if (mm == 0) continue;
+            const uint32_t m3 = static_cast<uint32_t>(__popcll(mm));
+            // per candidate j: lb_j = pool entries below it; per pool entry:
+            // nlt = candidates above it, so it moves up by m3 - nlt
             uint32_t lb = 0, rank = 0;
-            uint32_t shift[S];
+            uint32_t nlt[S];
 #pragma unroll
-            for (int s = 0; s < S; ++s) shift[s] = 0;
+            for (int s = 0; s < S; ++s) nlt[s] = 0;
             while (mm) {
                 const int j = static_cast<int>(__builtin_ctzll(mm));
                 mm &= mm - 1;
                 const uint64_t kj = readlane64(key, j);
                 uint32_t below = 0;
 #pragma unroll
-                for (int s = 0; s < S; ++s) below += static_cast<uint32_t>(__popcll(__ballot(P[s] < kj)));
-                if (static_cast<int>(lane) == j) lb = below;
+                for (int s = 0; s < S; ++s) {
+                    const bool lt = P[s] < kj;
+                    below += static_cast<uint32_t>(__popcll(__ballot(lt)));
+                    nlt[s] += lt ? 1u : 0u;
+                }
+                lb = static_cast<int>(lane) == j ? below : lb;
                 rank += kj < key ? 1u : 0u;
-#pragma unroll
-                for (int s = 0; s < S; ++s) shift[s] += kj < P[s] ? 1u : 0u;
             }
-            const uint32_t m3 = static_cast<uint32_t>(__popcll(__ballot(cand)));
             const uint32_t nsize = min(size + m3, ef);
+            uint32_t shift[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) shift[s] = m3 - nlt[s];
             // merge through LDS; entries that fall off the end go to the spare
             // slot EF_CAP, so the stores need no branch
 #pragma unroll
@@ -693,8 +718,8 @@ __device__ void h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
 }
 
 template <int EF_CAP, int CW>
-__global__ void __launch_bounds__(256) h2_search_kernel(H2Dev ix, H2Job job, uint32_t hash_bits,
-                                                         Layout lay) {
+__device__ __forceinline__ void h2_search_body(const H2Dev& ix, const H2Job& job,
+                                               uint32_t hash_bits, const Layout& lay) {
     extern __shared__ __align__(16) unsigned char lds[];
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wid = threadIdx.x / kWave;
@@ -713,6 +738,26 @@ __global__ void __launch_bounds__(256) h2_search_kernel(H2Dev ix, H2Job job, uin
         if (lane == 0) nx = sw + atomicAdd(job.counter, 1u);
         q = __shfl(nx, 0);
     }
+}
+
+// Occupancy variants: uncapped registers, and capped for 5 or 6 waves per
+// SIMD (a few spills traded for more waves to hide the record loads).
+template <int EF_CAP, int CW>
+__global__ void __launch_bounds__(256) h2_search_kernel(H2Dev ix, H2Job job, uint32_t hash_bits,
+                                                         Layout lay) {
+    h2_search_body<EF_CAP, CW>(ix, job, hash_bits, lay);
+}
+
+template <int EF_CAP, int CW>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5)))
+h2_search_kernel_w5(H2Dev ix, H2Job job, uint32_t hash_bits, Layout lay) {
+    h2_search_body<EF_CAP, CW>(ix, job, hash_bits, lay);
+}
+
+template <int EF_CAP, int CW>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6)))
+h2_search_kernel_w6(H2Dev ix, H2Job job, uint32_t hash_bits, Layout lay) {
+    h2_search_body<EF_CAP, CW>(ix, job, hash_bits, lay);
 }
 
 // Preparation (and one node's scan) per query, for parity checks.
@@ -890,7 +935,24 @@ int code_words(const H2Dev& ix) {
 }
 
 template <int EF_CAP>
-SearchFn search_for_cw(int cw) {
+SearchFn search_for_cw(int cw, int occ) {
+    if (occ == 5) {
+        switch (cw) {
+            case 1: return &h2_search_kernel_w5<EF_CAP, 1>;
+            case 2: return &h2_search_kernel_w5<EF_CAP, 2>;
+            case 4: return &h2_search_kernel_w5<EF_CAP, 4>;
+            default: return nullptr;
+        }
+    }
+    if (occ == 6) {
+        switch (cw) {
+            case 1: return &h2_search_kernel_w6<EF_CAP, 1>;
+            case 2: return &h2_search_kernel_w6<EF_CAP, 2>;
+            case 4: return &h2_search_kernel_w6<EF_CAP, 4>;
+            default: return nullptr;
+        }
+    }
+    if (occ != 0) return nullptr;
     switch (cw) {
         case 1: return &h2_search_kernel<EF_CAP, 1>;
         case 2: return &h2_search_kernel<EF_CAP, 2>;
@@ -902,13 +964,13 @@ SearchFn search_for_cw(int cw) {
     }
 }
 
-SearchFn select_search(int ef_cap, int cw) {
+SearchFn select_search(int ef_cap, int cw, int occ) {
     switch (ef_cap) {
-        case 64: return search_for_cw<64>(cw);
-        case 128: return search_for_cw<128>(cw);
-        case 256: return search_for_cw<256>(cw);
-        case 512: return search_for_cw<512>(cw);
-        case 1024: return search_for_cw<1024>(cw);
+        case 64: return search_for_cw<64>(cw, occ);
+        case 128: return search_for_cw<128>(cw, occ);
+        case 256: return search_for_cw<256>(cw, occ);
+        case 512: return search_for_cw<512>(cw, occ);
+        case 1024: return search_for_cw<1024>(cw, occ);
         default: return nullptr;
     }
 }
@@ -927,6 +989,8 @@ ProbeFn select_probe(int cw) {
 
 }  // namespace
 
+bool h2_occupancy_ok(const H2Dev& ix) { return code_words(ix) <= 4; }
+
 size_t h2_lds_bytes(const H2Dev& ix, const H2Cfg& cfg, uint32_t kmax) {
     const Layout l = h2_layout(ix.dim, ix.p, static_cast<uint32_t>(cfg.ef_cap),
                                static_cast<uint32_t>(cfg.hash_bits), kmax);
@@ -937,7 +1001,7 @@ int h2_groups(const H2Dev& ix, const H2Cfg& cfg, uint32_t kmax, uint32_t nq) {
     const int need = static_cast<int>((nq + cfg.waves_per_group - 1) / cfg.waves_per_group);
     int groups = cfg.groups;
     if (groups <= 0) {
-        SearchFn k = select_search(cfg.ef_cap, code_words(ix));
+        SearchFn k = select_search(cfg.ef_cap, code_words(ix), cfg.occupancy);
         int dev = 0, cus = 0, per_cu = 0;
         if (!k || hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -955,7 +1019,7 @@ int h2_groups(const H2Dev& ix, const H2Cfg& cfg, uint32_t kmax, uint32_t nq) {
 
 hipError_t h2_launch_search(const H2Dev& ix, const H2Job& job, const H2Cfg& cfg,
                             hipStream_t stream) {
-    SearchFn k = select_search(cfg.ef_cap, code_words(ix));
+    SearchFn k = select_search(cfg.ef_cap, code_words(ix), cfg.occupancy);
     if (!k || cfg.waves_per_group < 1 || cfg.waves_per_group > 4 || cfg.hash_bits < 6 ||
         cfg.hash_bits > 14 || job.kstride == 0)
         return hipErrorInvalidValue;

@@ -478,13 +478,19 @@ lqg_status lqg_set_launch(lqg_index* h, uint32_t ef, const lqg_launch* cfg) {
     if (!h) return fail(LQG_ERR_ARG, "null index");
     if (ef != 0 && ef_class(ef) == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024");
     if (cfg && (cfg->waves_per_group < 1 || cfg->waves_per_group > 4 || cfg->cache_bits < 4 ||
-                cfg->cache_bits > 14 ||
-                (cfg->occupancy != 0 && cfg->occupancy != 6 && cfg->occupancy != 8) ||
-                cfg->groups < 0))
+                cfg->cache_bits > 14 || cfg->groups < 0))
         return fail(LQG_ERR_ARG, "launch settings out of range");
     const bool fs = h->mode == LQG_MODE_FASTSCAN;
     if (fs && cfg && cfg->cache_bits < 6)
         return fail(LQG_ERR_ARG, "FastScan visited-table bits must be in [6, 14]");
+    if (cfg && !fs && cfg->occupancy != 0 && cfg->occupancy != 6 && cfg->occupancy != 8)
+        return fail(LQG_ERR_ARG, "SQ8 occupancy must be 0, 6 or 8");
+    if (cfg && fs && cfg->occupancy != 0 && cfg->occupancy != 5 && cfg->occupancy != 6)
+        return fail(LQG_ERR_ARG, "FastScan occupancy must be 0, 5 or 6");
+    if (cfg && fs && cfg->occupancy != 0 && h->h2_ready && !lqg::h2_occupancy_ok(h->h2dev()))
+        return fail(LQG_ERR_UNSUPPORTED,
+                    "FastScan occupancy caps need at most 4 code words per lane (P / lanes per "
+                    "neighbour <= 128 bits)");
     for (int slot = 0; slot < 5; ++slot) {
         const int cap = 64 << slot;
         if (ef != 0 && ef_class(ef) != cap) continue;
@@ -496,6 +502,7 @@ lqg_status lqg_set_launch(lqg_index* h, uint32_t ef, const lqg_launch* cfg) {
             lqg::H2Cfg c = lqg::h2_default_launch(cap);
             c.waves_per_group = cfg->waves_per_group;
             c.hash_bits = cfg->cache_bits;
+            c.occupancy = cfg->occupancy;
             c.groups = cfg->groups;
             h->h2_launch[slot] = c;
             h->h2_launch_set[slot] = true;
@@ -527,7 +534,7 @@ lqg_status lqg_get_launch(const lqg_index* h, uint32_t ef, lqg_launch* cfg) {
             h->h2_launch_set[slot] ? h->h2_launch[slot] : lqg::h2_default_launch(cls);
         cfg->waves_per_group = c.waves_per_group;
         cfg->cache_bits = c.hash_bits;
-        cfg->occupancy = 0;
+        cfg->occupancy = c.occupancy;
         cfg->groups = c.groups;
         return LQG_OK;
     }
@@ -556,39 +563,45 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
     HIP_TRY(hipMemcpy(dq.p, queries, nq * h->dim * 4, hipMemcpyHostToDevice));
     const int slot = lqg::class_slot(cls);
     if (h->mode == LQG_MODE_FASTSCAN) {
-        // waves per workgroup x LDS visited-table size
+        // register cap x waves per workgroup x LDS visited-table size
         const bool had = h->h2_launch_set[slot];
         const lqg::H2Cfg saved = h->h2_launch[slot];
         const lqg::H2Cfg base = lqg::h2_default_launch(cls);
         float best_ms = 1e30f;
         lqg::H2Cfg best_cfg = base;
-        const int wpg[] = {1, 2, 4};
-        for (int extra = 0; extra < 2; ++extra)
-            for (int wg : wpg) {
-                lqg::H2Cfg c = base;
-                c.waves_per_group = wg;
-                c.hash_bits = base.hash_bits + extra;
-                h->h2_launch[slot] = c;
-                h->h2_launch_set[slot] = true;
-                float ms_sum = 0.f;
-                bool ok = true;
-                for (int rep = 0; rep < 3 && ok; ++rep) {
-                    if (hipEventRecord(h->ev0, h->stream) != hipSuccess) ok = false;
-                    if (ok && lqg::run_batch(h, dq.p, nq, nullptr, nullptr, ef, k, LQG_ORDER_NEAREST,
-                                             dids.p, nullptr, nullptr, h->stream) != LQG_OK)
-                        ok = false;
-                    if (ok && hipEventRecord(h->ev1, h->stream) != hipSuccess) ok = false;
-                    if (ok && hipEventSynchronize(h->ev1) != hipSuccess) ok = false;
-                    float ms = 0.f;
-                    if (ok && hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess) ok = false;
-                    if (rep > 0) ms_sum += ms;
+        std::vector<lqg::H2Cfg> cands;
+        const int n_occ = h->h2_ready && lqg::h2_occupancy_ok(h->h2dev()) ? 3 : 1;
+        for (int oi = 0; oi < n_occ; ++oi)
+            for (int extra = -1; extra < 2; ++extra)
+                for (int wg : {1, 2, 4}) {
+                    lqg::H2Cfg c = base;
+                    c.occupancy = oi == 0 ? 0 : oi + 4;  // 0, 5, 6
+                    c.waves_per_group = wg;
+                    c.hash_bits = base.hash_bits + extra;
+                    if (c.hash_bits >= 6) cands.push_back(c);
                 }
-                (void)hipGetLastError();
-                if (ok && ms_sum / 2 < best_ms) {
-                    best_ms = ms_sum / 2;
-                    best_cfg = c;
-                }
+        for (const lqg::H2Cfg& c : cands) {
+            h->h2_launch[slot] = c;
+            h->h2_launch_set[slot] = true;
+            float ms_sum = 0.f;
+            bool ok = true;
+            for (int rep = 0; rep < 3 && ok; ++rep) {
+                if (hipEventRecord(h->ev0, h->stream) != hipSuccess) ok = false;
+                if (ok && lqg::run_batch(h, dq.p, nq, nullptr, nullptr, ef, k, LQG_ORDER_NEAREST,
+                                         dids.p, nullptr, nullptr, h->stream) != LQG_OK)
+                    ok = false;
+                if (ok && hipEventRecord(h->ev1, h->stream) != hipSuccess) ok = false;
+                if (ok && hipEventSynchronize(h->ev1) != hipSuccess) ok = false;
+                float ms = 0.f;
+                if (ok && hipEventElapsedTime(&ms, h->ev0, h->ev1) != hipSuccess) ok = false;
+                if (rep > 0) ms_sum += ms;
             }
+            (void)hipGetLastError();
+            if (ok && ms_sum / 2 < best_ms) {
+                best_ms = ms_sum / 2;
+                best_cfg = c;
+            }
+        }
         dq.release();
         dids.release();
         if (best_ms >= 1e30f) {
@@ -601,7 +614,7 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
         if (best) {
             best->waves_per_group = best_cfg.waves_per_group;
             best->cache_bits = best_cfg.hash_bits;
-            best->occupancy = 0;
+            best->occupancy = best_cfg.occupancy;
             best->groups = best_cfg.groups;
         }
         return LQG_OK;

@@ -89,6 +89,7 @@ struct H2Cfg {
     int waves_per_group; // 1..4
     int groups;          // persistent grid size (0 = fill the device)
     int hash_bits;       // log2 entries of the per-wave LDS visited table
+    int occupancy = 0;   // 0 = uncapped registers, 5 or 6 = capped for that many waves per SIMD
 };
 
 // Returns LDS bytes one workgroup needs for this configuration.
@@ -96,6 +97,9 @@ size_t search_lds_bytes(const DevIndex& ix, const LaunchCfg& cfg);
 
 // FastScan search launches (h2_search.hip).
 size_t h2_lds_bytes(const H2Dev& ix, const H2Cfg& cfg, uint32_t kmax);
+// Whether the register-capped variants (occupancy 5, 6) exist for this index
+// shape (at most 4 code words per lane).
+bool h2_occupancy_ok(const H2Dev& ix);
 int h2_groups(const H2Dev& ix, const H2Cfg& cfg, uint32_t kmax, uint32_t nq);
 hipError_t h2_launch_search(const H2Dev& ix, const H2Job& job, const H2Cfg& cfg,
                             hipStream_t stream);

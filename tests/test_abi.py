@@ -181,6 +181,14 @@ def test_fastscan_graph_and_serving_validation():
     sq = np.zeros(1, np.int32)
     assert L.lqg_h2_probe(ix._h, _lib.ptr(q), 0, None, _lib.ptr(lut), _lib.ptr(lw),
                           _lib.ptr(sq), None, None) == 1
+    # register caps: SQ8 takes 0, 6, 8 (FastScan's 5 is not an SQ8 setting)
+    for occ in (5, 7, -1):
+        with pytest.raises(_lib.LqgError) as e:
+            ix.set_launch(0, waves_per_group=2, cache_bits=10, occupancy=occ)
+        assert e.value.status == 1
+    ix.set_launch(0, waves_per_group=2, cache_bits=10, occupancy=6)
+    assert ix.get_launch(64)["occupancy"] == 6
+    ix.reset_launch(0)
     # graph capture: null output, zero queries, unknown order
     g = C.c_void_p()
     assert L.lqg_graph_create(ix._h, None, 1, None, None, 10, 10, 1, None, None, None,

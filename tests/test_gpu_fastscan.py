@@ -156,6 +156,44 @@ def test_search_matches_oracle(oracle_mod, tmp_path, shape):
         assert st["evaluations"] == int(e_st[:, 1].sum())
 
 
+@pytest.mark.parametrize("shape", [s for s in SHAPES if s[1] <= 256],
+                         ids=lambda s: f"n{s[0]}_d{s[1]}_deg{s[2]}")
+def test_occupancy_variants_match_oracle(oracle_mod, tmp_path, shape):
+    """The register-capped kernels (5 and 6 waves per SIMD, code words per lane
+    <= 4) give the oracle's ids, distances and counts, also with a visited
+    table small enough to move to device memory; P/lanes above 128 bits
+    rejects the caps."""
+    from liteqg_amd._lib import LqgError
+
+    O = oracle_mod
+    n, d, deg, kind, knn, dup = shape
+    arr, _ = make_random_index(n, d, deg, seed=d + deg, kind=kind, knn=knn, dup_rate=dup)
+    ix = _index_from_arrays(arr, tmp_path, "c")
+    ix.build_sidecar(signs=_signs(ix.padded_dim, d))
+    ix.set_mode("fastscan")
+    rng = np.random.default_rng(d + 1)
+    Q = (arr.raw[rng.integers(0, n, 40)] + rng.normal(0, 0.05, (40, d))).astype(np.float32)
+    cw = ix.padded_dim // 8 // (1 if deg >= 64 else 2) // 4
+    if cw > 4:
+        with pytest.raises(LqgError) as e:
+            ix.set_launch(0, waves_per_group=2, cache_bits=10, occupancy=5)
+        assert e.value.status == 6
+        return
+    for occ in (5, 6):
+        for ef, k, bits in ((64, 10, 10), (200, 20, 10), (256, 10, 6)):
+            ix.set_launch(0, waves_per_group=4, cache_bits=bits, occupancy=occ)
+            assert ix.get_launch(ef)["occupancy"] == occ
+            ids, dis, st = ix.search_batch(Q, k, ef=ef, return_distances=True, return_stats=True)
+            e_ids, e_dis, e_st = _oracle_h2(O, arr, ix, Q, ef, k)
+            assert np.array_equal(ids, e_ids), (occ, ef)
+            assert np.array_equal(_bits(dis), _bits(e_dis)), (occ, ef)
+            assert st["expansions"] == int(e_st[:, 0].sum())
+    with pytest.raises(LqgError) as e:
+        ix.set_launch(0, waves_per_group=2, cache_bits=10, occupancy=8)
+    assert e.value.status == 1
+    ix.reset_launch(0)
+
+
 def test_update_results_and_uneven_ef_k(oracle_mod, tmp_path):
     """ef below k runs update_results (qg.hpp:395-418), including rows of node
     0 from the zero-filled id snapshot; per-query ef and k in one launch."""
