@@ -36,6 +36,8 @@ EF_SWEEP = (20, 50, 100, 200, 500)
 FS_EF_SWEEP = (50, 100, 200, 300, 500, 700, 1000)
 SERVE_RATES = (100_000.0, 1_000_000.0, 4_000_000.0)
 SERVE_LANES, SERVE_BATCH, SERVE_SLOTS = 4, 2048, 16384
+SERVE_MIX_RATES = (5_000.0, 20_000.0, 50_000.0)
+SERVE_MIX_LANES, SERVE_MIX_BATCH = 4, 512
 TARGET_RECALL = 0.95
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "h1_search_kernel_c2_traffic.md")
 
@@ -172,6 +174,56 @@ def serving_leg(ix, Q, ef, k, args):
     return out
 
 
+def serving_mix_leg(args):
+    """The serving workload of its own configuration: 1,000,000 unit-norm
+    768-d vectors from a 2,048-centre mixture (centres N(0, I), sigma 0.25
+    before normalisation), degree 32, graph built on the device (build EF 400, 3
+    iterations); Poisson arrivals of single-query requests at 5k, 20k and
+    50k per second with per-request ef uniform in {64..512} and k in {10, 100}
+    (SQ8 path, order=nearest).  Reports p50/p99 latency and recall@10 (first
+    10 ids of each request against FP64 brute force) per rate, plus an
+    unthrottled run for the engine's ceiling."""
+    from liteqg_amd import datasets
+    from liteqg_amd.serving import Server, poisson_arrivals
+    from liteqg_amd.symphonyqg import Index
+
+    n, d, deg, nq = args.serve_n, 768, 32, args.serve_nq
+    t = time.perf_counter()
+    base, Q = datasets.make("unit_mix", n, d, nq, seed=args.seed, centres=2048, sigma=0.25,
+                            centre_law="normal")
+    ix = Index("QG", "L2", n, d, deg)
+    ix.build_index(base, 400, num_iter=3)
+    build_s = time.perf_counter() - t
+    gt = ground_truth(base, Q, 10)
+    del base
+    rng = np.random.default_rng(args.seed + 3)
+    ef = rng.integers(64, 513, nq).astype(np.uint32)
+    k = np.where(rng.random(nq) < 0.5, 10, 100).astype(np.uint32)
+    out = {"workload": f"unit_mix N={n} D={d} deg={deg} (2048 centres N(0, I), sigma 0.25, "
+                       f"normalised), "
+                       f"{nq} requests per rate, ef ~ U{{64..512}}, k ~ {{10, 100}}",
+           "data_and_build_s": round(build_s, 1), "rates": []}
+    log(f"serving mix: data + build {build_s:.1f}s")
+    with Server(ix, lanes=SERVE_MIX_LANES, max_batch=SERVE_MIX_BATCH, slots=SERVE_SLOTS,
+                ef_max=512, kmax=100) as srv:
+        srv.replay(Q[:1000], ef[:1000], k[:1000], np.zeros(1000, np.uint64))  # warm-up
+        for rate in SERVE_MIX_RATES + (None,):
+            arr = (np.zeros(nq, np.uint64) if rate is None
+                   else poisson_arrivals(nq, rate, seed=args.seed + 2))
+            lat, ids = srv.replay(Q, ef, k, arr)
+            span = float((arr.astype(np.float64) + lat).max() - float(arr[0])) / 1e9
+            row = {"rate_per_s": rate if rate is not None else "unthrottled",
+                   "p50_ms": round(float(np.percentile(lat, 50)) / 1e6, 4),
+                   "p99_ms": round(float(np.percentile(lat, 99)) / 1e6, 4),
+                   "achieved_qps": round(nq / span, 1),
+                   "recall_at_10": round(recall_at_k(ids[:, :10], gt, 10), 4)}
+            out["rates"].append(row)
+            log(f"serving mix {row}")
+        out["engine"] = dict(srv.stats(), lanes=SERVE_MIX_LANES, max_batch=SERVE_MIX_BATCH,
+                             slots=SERVE_SLOTS)
+    return out
+
+
 def fastscan_leg(ix, db, Q, gt, k, args):
     """The FastScan RaBitQ path (search_qg) on the same index and queries:
     device sidecar build, ef sweep to recall@10 >= 0.95, batched timing, and
@@ -280,7 +332,11 @@ def main():
     ap.add_argument("--ef", type=int, default=0, help="skip the sweep and use this ef")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fastscan", action="store_true", help="skip the FastScan RaBitQ leg")
-    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving legs")
+    ap.add_argument("--serve-n", type=int, default=1_000_000,
+                    help="base vectors of the 768-d serving workload")
+    ap.add_argument("--serve-nq", type=int, default=10_000,
+                    help="requests per arrival rate of the 768-d serving workload")
     ap.add_argument("--plain", type=int, default=500, help="queries timed on the plain path")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 runs: build, --probe-launches launches at --ef (default 50) with "
@@ -398,6 +454,8 @@ def main():
 
     fastscan = None if args.no_fastscan else fastscan_leg(ix, db, Q, gt, k, args)
 
+    serving_mix = None if args.no_serve else serving_mix_leg(args)
+
     cpu = None
     if not args.no_cpu:
         with tempfile.TemporaryDirectory() as tmp:
@@ -443,6 +501,8 @@ def main():
                             "qps": round(args.nq / (graph_ms / 1e3), 1)}
     if serving:
         line["serving"] = serving
+    if serving_mix:
+        line["serving_mix"] = serving_mix
     if fastscan:
         line["fastscan"] = fastscan
     if cpu:

@@ -99,6 +99,26 @@ def test_build_clustered_int_data(oracle_mod):
     assert recall(oracle_mod, ix, base, Q, 100) > 0.95
 
 
+@pytest.mark.parametrize("kind,d,deg,kw", [
+    ("unit_mix", 768, 32, dict(centres=64, sigma=0.25, centre_law="normal")),
+    ("pos_mix", 960, 64, dict(centres=32, sigma=0.05)),
+])
+def test_build_high_dimension(oracle_mod, tmp_path, kind, d, deg, kw):
+    """The 768-d and 960-d processes, shrunk: the device build's graph
+    searched at ef 100 reaches recall@10 0.9 against FP64 brute force, and the
+    device search on it equals the oracle's."""
+    from liteqg_amd import datasets
+
+    base, Q = datasets.make(kind, 6000, d, 100, seed=4, **kw)
+    ix = built_index(base, deg=deg, ef=200)
+    check_rows(ix.graph(), 6000)
+    assert recall(oracle_mod, ix, base, Q, 100) > 0.9
+    ix.save(str(tmp_path / "h.index"))
+    arr = oracle_mod.read_index(str(tmp_path / "h.index"), 6000, d, deg)
+    e_ids, _, _ = oracle_mod.search_canon_batch(arr, Q[:30], 100, 10, "nearest")
+    assert np.array_equal(ix.search_batch(Q[:30], 10, ef=100, order="nearest"), e_ids)
+
+
 def test_build_argument_errors():
     from liteqg_amd.symphonyqg import Index, LqgError
 

@@ -95,6 +95,8 @@ CASES = [
     (3000, 64, 32, 300, 10, "uniform", False, 0.0),    # random graph
     (3000, 64, 32, 100, 10, "uniform", True, 0.3),     # repeated ids in lists
     (2000, 48, 96, 70, 10, "uniform", True, 0.0),      # deg > 64: two id chunks
+    (3000, 768, 32, 300, 100, "uniform", True, 0.0),   # 768-d rows (several chunk loads)
+    (2000, 960, 64, 500, 100, "uniform", True, 0.0),   # 960-d rows, degree 64
 ]
 
 
