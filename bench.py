@@ -262,12 +262,36 @@ def fastscan_leg(ix, db, Q, gt, k, args):
     rec_bytes = 4 * d + deg * p // 8 + 16 * deg
     alg = float(np.sum(4 * d + E * rec_bytes + 4 * k))
     achieved = alg / (ms_step / 1e3) / 1e9
+    # the matrix-core rotation: its error against FP64 (the FHT's beside it),
+    # then the same search with GEMM-rotated queries
+    import scipy.linalg
+
+    qs = Q[:1000]
+    signs = ix.get_sidecar(signs_only=True)
+    qpad = np.zeros((qs.shape[0], p))
+    qpad[:, :d] = qs.astype(np.float64) * signs[:d].astype(np.float64)
+    exact = qpad @ scipy.linalg.hadamard(p).astype(np.float64).T
+    scale = np.abs(exact).max(axis=1, keepdims=True)
+    rot_err = {m: float((np.abs(ix.rotate_queries(qs, m) - exact) / scale).max())
+               for m in ("fht", "gemm")}
+    ids_fht = db.results()
+    ix.set_rotation("gemm")
+    ms_gemm, _ = time_launches(db, chosen, args.steps, args.warmup)
+    ids_gemm = db.results()
+    ix.set_rotation("fht")
+    gemm = {"rel_err_vs_fp64": rot_err["gemm"], "fht_rel_err_vs_fp64": rot_err["fht"],
+            "ms_per_step": round(ms_gemm, 4), "qps": round(args.nq / ms_gemm * 1e3, 1),
+            "recall_at_10": round(recall_at_k(ids_gemm, gt, k), 4),
+            "ids_overlap_with_fht": round(float(np.mean(
+                [len(set(a) & set(b)) / len(a) for a, b in zip(ids_fht, ids_gemm)])), 5)}
+    log(f"fastscan gemm rotation: {gemm}")
     ix.set_mode("sq8")
     log(f"fastscan timed ef={chosen}: {ms_step:.3f} ms/step, {qps:.0f} QPS, "
         f"{achieved:.0f} GB/s")
     return {"path": "FastScan RaBitQ search_qg (1-bit codes, exact re-rank)", "ef": chosen,
             "recall_at_10": rec, "qps": round(qps, 1), "ms_per_step": round(ms_step, 4),
             "ef_sweep": sweep, "sidecar_build_s": round(side_s, 2), "launch": tuned,
+            "gemm_rotation": gemm,
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "bytes_per_query": round(alg / args.nq, 1),

@@ -50,6 +50,21 @@ typedef enum {
     LQG_MODE_FASTSCAN = 1
 } lqg_mode;
 
+/* How the FastScan path rotates each query (QGQuery::query_prepare,
+ * qg_query.hpp:28-43, via FHTRotator::rotate, utils/rotator.hpp:88-114). */
+typedef enum {
+    /* In the search kernel: sign multiply, then the unnormalised
+     * Walsh-Hadamard butterflies in the reference's order.  Bit-identical
+     * rotated queries, hence LUTs, to the reference's (default). */
+    LQG_ROTATION_FHT = 0,
+    /* One batched FP32 GEMM on the matrix cores before the search:
+     * Q[nq x D] times H_P diag(signs).  Same products, another summation
+     * order, so rotated values differ from the FHT's in the last bits and a
+     * LUT entry can differ by one where a value sits on a quantisation
+     * boundary. */
+    LQG_ROTATION_GEMM = 1
+} lqg_rotation;
+
 typedef struct lqg_index lqg_index;
 
 /* Per-call counters (optional out-parameter of the batch entry points). */
@@ -261,6 +276,17 @@ lqg_status lqg_h2_get_sidecar(const lqg_index* index, float* signs, uint8_t* cod
  * checks the header against the loaded index. */
 lqg_status lqg_h2_save_sidecar(const lqg_index* index, const char* path);
 lqg_status lqg_h2_load_sidecar(lqg_index* index, const char* path);
+
+/* Rotation used by the FastScan searches of this index (default FHT).  No
+ * counterpart in the reference, which has only the FHT. */
+lqg_status lqg_h2_set_rotation(lqg_index* index, lqg_rotation method);
+lqg_status lqg_h2_get_rotation(const lqg_index* index, lqg_rotation* method);
+
+/* The rotated queries (nq x P floats, host memory in and out) by either
+ * method, for checks against an FP64 rotation.  Needs a sidecar (the signs).
+ * Replaces FHTRotator::rotate (utils/rotator.hpp:88-114) for a batch. */
+lqg_status lqg_h2_rotate(lqg_index* index, const float* queries, uint64_t nq,
+                         lqg_rotation method, float* out);
 
 /* Parity probe of the FastScan path: for each of nq queries (host memory) the
  * device's query preparation (QGQuery::query_prepare, qg_query.hpp:28-43):

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "liblqg.so")
 LQG_OK = 0
 ORDER = {"reference": 0, "nearest": 1}
 MODE = {"sq8": 0, "fastscan": 1}
+ROTATION = {"fht": 0, "gemm": 1}
 
 
 class LqgError(RuntimeError):
@@ -75,6 +76,9 @@ _SIGS = {
     "lqg_h2_get_sidecar": [_P, _P, _P, _P],
     "lqg_h2_save_sidecar": [_P, C.c_char_p],
     "lqg_h2_load_sidecar": [_P, C.c_char_p],
+    "lqg_h2_set_rotation": [_P, C.c_int],
+    "lqg_h2_get_rotation": [_P, _P],
+    "lqg_h2_rotate": [_P, _P, C.c_uint64, C.c_int, _P],
     "lqg_h2_probe": [_P, _P, C.c_uint64, _P, _P, _P, _P, _P, _P],
     "lqg_graph_create": [_P, _P, C.c_uint64, _P, _P, C.c_uint32, C.c_uint32, C.c_int, _P, _P, _P,
                          C.POINTER(_P)],

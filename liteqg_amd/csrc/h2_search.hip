@@ -125,22 +125,34 @@ struct Prep {
     int32_t sumq;
 };
 
-__device__ Prep prepare(const H2Dev& ix, const float* qsrc, const H2Lds& w, uint32_t lpv,
-                        uint32_t lane) {
-    const uint32_t dim = ix.dim, p = ix.p;
-    for (uint32_t i = lane; i < dim; i += kWave) w.qv[i] = qsrc[i];
-    for (uint32_t i = lane; i < p; i += kWave) w.tb[i] = i < dim ? qsrc[i] * ix.signs[i] : 0.0f;
-    wave_sync();
-    // unnormalised Walsh-Hadamard transform, strides 1, 2, ..., p/2
+// In-place unnormalised Walsh-Hadamard transform of t[0..p) in LDS by one
+// wave, butterflies (a, b) -> (a + b, a - b) with strides 1, 2, ..., p/2.
+__device__ void fht_lds(float* t, uint32_t p, uint32_t lane) {
     for (uint32_t lh = 0; (1u << lh) < p; ++lh) {
         const uint32_t h = 1u << lh;
         for (uint32_t b = lane; b < p / 2; b += kWave) {
             const uint32_t j = ((b >> lh) << (lh + 1)) | (b & (h - 1u));
-            const float a = w.tb[j], c = w.tb[j + h];
-            w.tb[j] = a + c;
-            w.tb[j + h] = a - c;
+            const float a = t[j], c = t[j + h];
+            t[j] = a + c;
+            t[j + h] = a - c;
         }
         wave_sync();
+    }
+}
+
+// rsrc: the query already rotated (p floats, the GEMM rotation), or null to
+// rotate here with the FHT.
+__device__ Prep prepare(const H2Dev& ix, const float* qsrc, const float* rsrc, const H2Lds& w,
+                        uint32_t lpv, uint32_t lane) {
+    const uint32_t dim = ix.dim, p = ix.p;
+    for (uint32_t i = lane; i < dim; i += kWave) w.qv[i] = qsrc[i];
+    if (rsrc) {
+        for (uint32_t i = lane; i < p; i += kWave) w.tb[i] = rsrc[i];
+        wave_sync();
+    } else {
+        for (uint32_t i = lane; i < p; i += kWave) w.tb[i] = i < dim ? qsrc[i] * ix.signs[i] : 0.0f;
+        wave_sync();
+        fht_lds(w.tb, p, lane);
     }
     // data_range (utils/scalar_quantize.hpp:11-38): min and max seeded with 0
     float lo = 0.0f, hi = 0.0f;
@@ -497,7 +509,9 @@ __device__ void h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
     const uint32_t part = lane % lpv;
     const uint8_t* lut_part = w.lut + 16u * cpl * part + (part ? 16u : 0u);
 
-    const Prep pr = prepare(ix, job.queries + static_cast<size_t>(q) * dim, w, lpv, lane);
+    const Prep pr = prepare(ix, job.queries + static_cast<size_t>(q) * dim,
+                            job.rot ? job.rot + static_cast<size_t>(q) * ix.p : nullptr, w, lpv,
+                            lane);
 
     Visited vis;
     vis.lds = w.hv;
@@ -772,7 +786,7 @@ __global__ void __launch_bounds__(64) h2_probe_kernel(H2Dev ix, const float* que
     const H2Lds w = carve(lds, lay);
     const uint32_t deg = ix.deg;
     const uint32_t lpv = deg >= 64 ? 1u : 2u;
-    const Prep pr = prepare(ix, queries + static_cast<size_t>(q) * ix.dim, w, lpv, lane);
+    const Prep pr = prepare(ix, queries + static_cast<size_t>(q) * ix.dim, nullptr, w, lpv, lane);
     const uint32_t cpl = (ix.p / 4) / lpv;
     // LUT in the reference's layout (no bank skew)
     for (uint32_t idx = lane; idx < 4 * ix.p; idx += kWave) {
@@ -826,17 +840,69 @@ __global__ void __launch_bounds__(256) h2_rotate_kernel(H2Dev ix, float* rot) {
     const float* src = reinterpret_cast<const float*>(ix.rec + static_cast<size_t>(u) * ix.stride);
     for (uint32_t i = lane; i < ix.p; i += kWave) t[i] = i < ix.dim ? src[i] * ix.signs[i] : 0.0f;
     wave_sync();
-    for (uint32_t lh = 0; (1u << lh) < ix.p; ++lh) {
-        const uint32_t h = 1u << lh;
-        for (uint32_t b = lane; b < ix.p / 2; b += kWave) {
-            const uint32_t j = ((b >> lh) << (lh + 1)) | (b & (h - 1u));
-            const float a = t[j], c = t[j + h];
-            t[j] = a + c;
-            t[j + h] = a - c;
-        }
-        wave_sync();
-    }
+    fht_lds(t, ix.p, lane);
     for (uint32_t i = lane; i < ix.p; i += kWave) rot[static_cast<size_t>(u) * ix.p + i] = t[i];
+}
+
+// Queries rotated by the FHT (one wave per query): what prepare computes.
+__global__ void __launch_bounds__(256) h2_rotq_fht_kernel(H2Dev ix, const float* queries,
+                                                           uint32_t nq, float* out) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wid = threadIdx.x / kWave;
+    const uint32_t q = blockIdx.x * (blockDim.x / kWave) + wid;
+    if (q >= nq) return;
+    float* t = reinterpret_cast<float*>(lds) + static_cast<size_t>(wid) * ix.p;
+    const float* src = queries + static_cast<size_t>(q) * ix.dim;
+    for (uint32_t i = lane; i < ix.p; i += kWave) t[i] = i < ix.dim ? src[i] * ix.signs[i] : 0.0f;
+    wave_sync();
+    fht_lds(t, ix.p, lane);
+    for (uint32_t i = lane; i < ix.p; i += kWave) out[static_cast<size_t>(q) * ix.p + i] = t[i];
+}
+
+// Queries rotated by one FP32 GEMM on the matrix cores: out[nq x p] =
+// Q[nq x dim] M[dim x p] with M[i][j] = signs[i] (-1)^popcount(i & j), the
+// FHT's matrix (H_p diag(signs), Sylvester order), generated in registers.
+// Each wave computes a 16-query x 64-column block with
+// v_mfma_f32_16x16x4_f32: lane l supplies A[l % 16][k0 + l / 16] and
+// B[k0 + l / 16][l % 16] per step of 4 and holds D[4 (l / 16) + r][l % 16],
+// r = 0..3.  Same products as the FHT, different summation order: the
+// rounding differs from the reference's rotation by a few ulps.
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) h2_rotq_gemm_kernel(H2Dev ix, const float* queries,
+                                                            uint32_t nq, float* out) {
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t gw = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const uint32_t ct_n = ix.p / 64;
+    const uint32_t qt = gw / ct_n, ct = gw % ct_n;
+    if (qt * 16 >= nq) return;
+    const uint32_t li = lane & 15u, lk = lane >> 4;
+    const uint32_t row = qt * 16 + li;
+    const float* qrow = queries + static_cast<size_t>(row < nq ? row : 0) * ix.dim;
+    floatx4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t k0 = 0; k0 < ix.dim; k0 += 4) {
+        const uint32_t k = k0 + lk;
+        const bool in = k < ix.dim;
+        const float a = in && row < nq ? qrow[k] : 0.0f;
+        const float sk = in ? ix.signs[k] : 0.0f;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t j = ct * 64 + 16 * static_cast<uint32_t>(t) + li;
+            const float b = (__popc(k & j) & 1u) ? -sk : sk;
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t orow = qt * 16 + 4 * lk + static_cast<uint32_t>(r);
+            const uint32_t col = ct * 64 + 16 * static_cast<uint32_t>(t) + li;
+            if (orow < nq) out[static_cast<size_t>(orow) * ix.p + col] = acc[t][r];
+        }
 }
 
 // One thread per (node u, neighbour slot j): residual o = R x_j - R x_u, sign
@@ -1061,6 +1127,21 @@ hipError_t h2_launch_sidecar(const H2Dev& ix, float* rot, hipStream_t stream) {
     const uint64_t threads = static_cast<uint64_t>(ix.n) * ix.deg;
     hipLaunchKernelGGL(h2_sidecar_kernel, dim3(static_cast<uint32_t>((threads + 255) / 256)),
                        dim3(256), 0, stream, ix, rot, fn);
+    return hipGetLastError();
+}
+
+hipError_t h2_launch_rotate_queries(const H2Dev& ix, const float* queries, uint32_t nq,
+                                    int method, float* out, hipStream_t stream) {
+    if (nq == 0 || ix.p < 64 || (method != 0 && method != 1)) return hipErrorInvalidValue;
+    const uint32_t waves = 4;
+    if (method == 0) {
+        hipLaunchKernelGGL(h2_rotq_fht_kernel, dim3((nq + waves - 1) / waves), dim3(kWave * waves),
+                           static_cast<size_t>(waves) * ix.p * 4, stream, ix, queries, nq, out);
+    } else {
+        const uint64_t tiles = static_cast<uint64_t>((nq + 15) / 16) * (ix.p / 64);
+        hipLaunchKernelGGL(h2_rotq_gemm_kernel, dim3(static_cast<uint32_t>((tiles + waves - 1) / waves)),
+                           dim3(kWave * waves), 0, stream, ix, queries, nq, out);
+    }
     return hipGetLastError();
 }
 

@@ -181,6 +181,11 @@ lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const
     LQG_HIP_TRY(sc->gvis.reserve((static_cast<size_t>(groups) * cfg.waves_per_group) << kGvisBits));
     LQG_HIP_TRY(sc->err.reserve(1));
     LQG_HIP_TRY(sc->counter.reserve(1));
+    if (h->h2_rotation == LQG_ROTATION_GEMM) {
+        LQG_HIP_TRY(sc->rot.reserve(nq * h->h2_p));
+        LQG_HIP_TRY(h2_launch_rotate_queries(ix, d_queries, static_cast<uint32_t>(nq), 1,
+                                             sc->rot.p, s));
+    }
     H2Job job;
     job.queries = d_queries;
     job.ef = d_ef;
@@ -196,6 +201,7 @@ lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const
     job.gvis = sc->gvis.p;
     job.gvis_bits = kGvisBits;
     job.err = sc->err.p;
+    job.rot = h->h2_rotation == LQG_ROTATION_GEMM ? sc->rot.p : nullptr;
     LQG_HIP_TRY(launch_reset_counter(sc->counter.p, s));
     LQG_HIP_TRY(launch_reset_counter(sc->err.p, s));
     LQG_HIP_TRY(h2_launch_search(ix, job, cfg, s));
@@ -427,6 +433,41 @@ lqg_status lqg_h2_load_sidecar(lqg_index* h, const char* path) {
     fclose(f);
     if (!ok) return fail(LQG_ERR_IO, std::string("short read from ") + path);
     h->h2_ready = true;
+    return LQG_OK;
+}
+
+lqg_status lqg_h2_set_rotation(lqg_index* h, lqg_rotation method) {
+    if (!h) return fail(LQG_ERR_ARG, "null index");
+    if (method != LQG_ROTATION_FHT && method != LQG_ROTATION_GEMM)
+        return fail(LQG_ERR_ARG, "rotation must be LQG_ROTATION_FHT or LQG_ROTATION_GEMM");
+    h->h2_rotation = method;
+    return LQG_OK;
+}
+
+lqg_status lqg_h2_get_rotation(const lqg_index* h, lqg_rotation* method) {
+    if (!h || !method) return fail(LQG_ERR_ARG, "null argument");
+    *method = h->h2_rotation;
+    return LQG_OK;
+}
+
+lqg_status lqg_h2_rotate(lqg_index* h, const float* queries, uint64_t nq, lqg_rotation method,
+                         float* out) {
+    if (!h || !queries || !out) return fail(LQG_ERR_ARG, "null argument");
+    if (method != LQG_ROTATION_FHT && method != LQG_ROTATION_GEMM)
+        return fail(LQG_ERR_ARG, "rotation must be LQG_ROTATION_FHT or LQG_ROTATION_GEMM");
+    if (nq == 0 || nq > 0xFFFFFFFFull) return fail(LQG_ERR_ARG, "nq must be in [1, 2^32)");
+    if (!h->h2_ready) return fail(LQG_ERR_STATE, "no sidecar (rotation signs)");
+    for (uint64_t i = 0; i < nq * h->dim; ++i)
+        if (!std::isfinite(queries[i])) return fail(LQG_ERR_ARG, "query contains a non-finite value");
+    lqg::DevBuf<float> dq, dout;
+    LQG_HIP_TRY(dq.reserve(nq * h->dim));
+    LQG_HIP_TRY(dout.reserve(nq * h->h2_p));
+    LQG_HIP_TRY(hipMemcpy(dq.p, queries, nq * h->dim * 4, hipMemcpyHostToDevice));
+    LQG_HIP_TRY(lqg::h2_launch_rotate_queries(h->h2dev(), dq.p, static_cast<uint32_t>(nq),
+                                              method == LQG_ROTATION_GEMM ? 1 : 0, dout.p,
+                                              h->stream));
+    LQG_HIP_TRY(hipStreamSynchronize(h->stream));
+    LQG_HIP_TRY(hipMemcpy(out, dout.p, nq * h->h2_p * 4, hipMemcpyDeviceToHost));
     return LQG_OK;
 }
 

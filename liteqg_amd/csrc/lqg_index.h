@@ -49,10 +49,12 @@ struct Scratch {
     DevBuf<uint32_t> counter;  // work-queue head
     DevBuf<uint32_t> err;      // FastScan visited-set overflow flag
     DevBuf<uint32_t> gvis;     // FastScan per-wave overflow visited tables
+    DevBuf<float> rot;         // FastScan queries rotated by the GEMM (nq x P)
     void release() {
         counter.release();
         err.release();
         gvis.release();
+        rot.release();
     }
 };
 
@@ -109,6 +111,7 @@ struct lqg_index {
     lqg::DevBuf<float> d_h2signs;
     lqg::H2Cfg h2_launch[5];
     bool h2_launch_set[5] = {false, false, false, false, false};
+    lqg_rotation h2_rotation = LQG_ROTATION_FHT;
 
     lqg::H2Dev h2dev() const {
         lqg::H2Dev ix;

@@ -82,6 +82,7 @@ struct H2Job {
     uint32_t* gvis;         // per-wave overflow visited tables (gvis_bits each)
     uint32_t gvis_bits;
     uint32_t* err;          // set to 1 if a visited set overflows both tables
+    const float* rot = nullptr;  // nq x p queries rotated beforehand (GEMM), or null (FHT)
 };
 
 struct H2Cfg {
@@ -97,6 +98,10 @@ size_t search_lds_bytes(const DevIndex& ix, const LaunchCfg& cfg);
 
 // FastScan search launches (h2_search.hip).
 size_t h2_lds_bytes(const H2Dev& ix, const H2Cfg& cfg, uint32_t kmax);
+// Rotated queries (nq x p): method 0 = the FHT of the search kernel (one
+// wave per query, bit-identical to it), 1 = one FP32 GEMM on the matrix cores.
+hipError_t h2_launch_rotate_queries(const H2Dev& ix, const float* queries, uint32_t nq,
+                                    int method, float* out, hipStream_t stream);
 // Whether the register-capped variants (occupancy 5, 6) exist for this index
 // shape (at most 4 code words per lane).
 bool h2_occupancy_ok(const H2Dev& ix);

@@ -29,7 +29,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import MODE, ORDER, Launch, LqgError, Stats, check, lib, ptr
+from ._lib import MODE, ORDER, ROTATION, Launch, LqgError, Stats, check, lib, ptr
 
 UINT_MAX = 0xFFFFFFFF
 
@@ -207,10 +207,14 @@ class Index:
         check(lib().lqg_h2_set_sidecar(self._h, ptr(s), ptr(c), 0 if layout == "blocks" else 1,
                                        ptr(f)))
 
-    def get_sidecar(self):
-        """(signs, codes vector-major, factors) of the installed sidecar."""
+    def get_sidecar(self, signs_only: bool = False):
+        """(signs, codes vector-major, factors) of the installed sidecar, or
+        only the signs."""
         p, g, n = self.padded_dim, self.degree_bound, self.num_elements
         s = np.empty(p, np.float32)
+        if signs_only:
+            check(lib().lqg_h2_get_sidecar(self._h, ptr(s), None, None))
+            return s
         c = np.empty((n, g * p // 8), np.uint8)
         f = np.empty((n, 3 * g), np.float32)
         check(lib().lqg_h2_get_sidecar(self._h, ptr(s), ptr(c), ptr(f)))
@@ -221,6 +225,28 @@ class Index:
 
     def load_sidecar(self, path: str) -> None:
         check(lib().lqg_h2_load_sidecar(self._h, str(path).encode()))
+
+    def set_rotation(self, method: str) -> None:
+        """FastScan query rotation: ``"fht"`` (in the search kernel, the
+        reference's rotation bit for bit; default) or ``"gemm"`` (one FP32 GEMM
+        on the matrix cores before the search)."""
+        if method not in ROTATION:
+            raise ValueError(f"rotation must be 'fht' or 'gemm', got {method!r}")
+        check(lib().lqg_h2_set_rotation(self._h, ROTATION[method]))
+
+    def get_rotation(self) -> str:
+        v = C.c_int()
+        check(lib().lqg_h2_get_rotation(self._h, C.byref(v)))
+        return {c: n for n, c in ROTATION.items()}[v.value]
+
+    def rotate_queries(self, queries, method: str = "fht") -> np.ndarray:
+        """The rotated queries (nq x P float32) by either method."""
+        if method not in ROTATION:
+            raise ValueError(f"rotation must be 'fht' or 'gemm', got {method!r}")
+        Q = np.ascontiguousarray(queries, dtype=np.float32).reshape(-1, self.dimension)
+        out = np.empty((Q.shape[0], self.padded_dim), np.float32)
+        check(lib().lqg_h2_rotate(self._h, ptr(Q), Q.shape[0], ROTATION[method], ptr(out)))
+        return out
 
     def fastscan_probe(self, queries, nodes=None):
         """Device query preparation (and, with nodes, the exact distance to

@@ -181,6 +181,15 @@ def test_fastscan_graph_and_serving_validation():
     sq = np.zeros(1, np.int32)
     assert L.lqg_h2_probe(ix._h, _lib.ptr(q), 0, None, _lib.ptr(lut), _lib.ptr(lw),
                           _lib.ptr(sq), None, None) == 1
+    # rotation method
+    assert L.lqg_h2_set_rotation(ix._h, 2) == 1
+    with pytest.raises(ValueError):
+        ix.set_rotation("qr")
+    ix.set_rotation("gemm")
+    assert ix.get_rotation() == "gemm"
+    ix.set_rotation("fht")
+    assert L.lqg_h2_rotate(ix._h, _lib.ptr(q), 1, 0, _lib.ptr(q)) == 2  # no sidecar yet
+    assert L.lqg_h2_rotate(ix._h, _lib.ptr(q), 1, 3, _lib.ptr(q)) == 1
     # register caps: SQ8 takes 0, 6, 8 (FastScan's 5 is not an SQ8 setting)
     for occ in (5, 7, -1):
         with pytest.raises(_lib.LqgError) as e:

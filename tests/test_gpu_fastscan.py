@@ -194,6 +194,55 @@ def test_occupancy_variants_match_oracle(oracle_mod, tmp_path, shape):
     ix.reset_launch(0)
 
 
+# Max |GEMM - FP64| over max |FP64| of each rotated query.  FP32 sums of D
+# terms, each at most max|q| / sqrt(P): the rounding stays within a small
+# multiple of sqrt(D) ulps of the largest value, far below this bound; a wrong
+# operand layout or sign pattern gives errors of order 1.
+GEMM_ROT_TOL = 1e-5
+
+
+@pytest.mark.parametrize("shape", [(1500, 64, 32), (1200, 128, 32), (800, 200, 32),
+                                   (500, 768, 32), (400, 960, 64)],
+                         ids=lambda s: f"n{s[0]}_d{s[1]}_deg{s[2]}")
+def test_rotation_fht_and_gemm(oracle_mod, tmp_path, shape):
+    """Query rotation: the device FHT equals the oracle's (the reference's
+    FHTRotator, bit for bit); the MFMA GEMM is within GEMM_ROT_TOL of an FP64
+    rotation H_P diag(signs) q; a FastScan search with GEMM-rotated queries
+    finds at least 98% of the FHT search's ids (mean per-query overlap)."""
+    import scipy.linalg
+
+    O = oracle_mod
+    n, d, deg = shape
+    arr, _ = make_random_index(n, d, deg, seed=d + 7, knn=True)
+    ix = _index_from_arrays(arr, tmp_path, "rot")
+    p = ix.padded_dim
+    signs = _signs(p, d + 1)
+    ix.build_sidecar(signs=signs)
+    rng = np.random.default_rng(d)
+    Q = (arr.raw[rng.integers(0, n, 37)] + rng.normal(0, 0.05, (37, d))).astype(np.float32)
+    fht = ix.rotate_queries(Q, "fht")
+    gemm = ix.rotate_queries(Q, "gemm")
+    assert fht.shape == (37, p) and gemm.shape == (37, p)
+    for i in range(Q.shape[0]):
+        assert np.array_equal(_bits(fht[i]), _bits(O.fht_rotate(signs, Q[i], p)))
+    H = scipy.linalg.hadamard(p).astype(np.float64)
+    qpad = np.zeros((Q.shape[0], p))
+    qpad[:, :d] = Q.astype(np.float64) * signs[:d].astype(np.float64)
+    exact = qpad @ H.T
+    scale = np.abs(exact).max(axis=1, keepdims=True)
+    err_gemm = float((np.abs(gemm - exact) / scale).max())
+    err_fht = float((np.abs(fht - exact) / scale).max())
+    assert err_gemm <= GEMM_ROT_TOL, (err_gemm, err_fht)
+    ix.set_mode("fastscan")
+    ids_fht = ix.search_batch(Q, 10, ef=128)
+    ix.set_rotation("gemm")
+    assert ix.get_rotation() == "gemm"
+    ids_gemm = ix.search_batch(Q, 10, ef=128)
+    ix.set_rotation("fht")
+    overlap = np.mean([len(set(a) & set(b)) / len(a) for a, b in zip(ids_fht, ids_gemm)])
+    assert overlap >= 0.98, (overlap, err_gemm, err_fht)
+
+
 def test_update_results_and_uneven_ef_k(oracle_mod, tmp_path):
     """ef below k runs update_results (qg.hpp:395-418), including rows of node
     0 from the zero-filled id snapshot; per-query ef and k in one launch."""
