@@ -37,9 +37,10 @@ FS_EF_SWEEP = (50, 100, 200, 300, 500, 700, 1000)
 SERVE_RATES = (100_000.0, 1_000_000.0, 4_000_000.0)
 SERVE_LANES, SERVE_BATCH, SERVE_SLOTS = 4, 2048, 16384
 SERVE_MIX_RATES = (5_000.0, 20_000.0, 50_000.0)
-SERVE_MIX_LANES, SERVE_MIX_BATCH = 4, 512
+SERVE_MIX_LANES, SERVE_MIX_BATCH = 4, 2048
 TARGET_RECALL = 0.95
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "h1_search_kernel_c2_traffic.md")
+TRAFFIC_FILE_FS = os.path.join(ROOT, "profiles", "h2_search_kernel_c2_traffic.md")
 
 
 def log(*a):
@@ -294,17 +295,18 @@ def fastscan_leg(ix, db, Q, gt, k, args):
             "gemm_rotation": gemm,
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": read_traffic(TRAFFIC_FILE_FS),
                          "bytes_per_query": round(alg / args.nq, 1),
                          "record_bytes": rec_bytes,
                          "expansions_per_query": round(float(E.mean()), 2),
                          "pops_per_query": round(float(pops.mean()), 2)}}
 
 
-def read_traffic():
+def read_traffic(path=TRAFFIC_FILE):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
-    if not os.path.exists(TRAFFIC_FILE):
+    if not os.path.exists(path):
         return None
-    for line in open(TRAFFIC_FILE):
+    for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
