@@ -179,12 +179,14 @@ __device__ __forceinline__ uint64_t eval_fresh(const DevIndex& ix, const float* 
 // sum (q_i - c_i)^2 whatever the order.  That integer is computed here as
 // sum q^2 + sum c^2 - 2 sum q*c with v_dot4_u32_u8 and converted to float
 // (exact below 2^24): bit-identical.  Candidate c's key is returned on lane 2c.
+template <int NCH>
 __device__ __forceinline__ uint64_t eval_fresh_int(const DevIndex& ix, const uint32_t* qp,
                                                    uint32_t qq, const uint32_t* bids, uint32_t m,
                                                    uint32_t lane) {
     // two lanes per candidate (m <= 32): lane 2c + h takes half h of the row's
-    // 16-byte chunks (at most 8 of the 16 a 256-byte row has); the integer
-    // partial sums are combined with one shuffle
+    // 16-byte chunks (at most NCH: 8 of the 16 a 256-byte row has, or 4 for
+    // rows of at most 128 bytes, which then need half the registers); the
+    // integer partial sums are combined with one shuffle
     const uint32_t c = lane >> 1, h = lane & 1u;
     const bool act = c < m;
     const uint32_t n16 = ix.dim >> 4;
@@ -196,12 +198,12 @@ __device__ __forceinline__ uint64_t eval_fresh_int(const DevIndex& ix, const uin
         id = bids[c];
         const uint4* row =
             reinterpret_cast<const uint4*>(ix.qdata + static_cast<size_t>(id) * ix.dim) + c0;
-        uint4 w[8];
+        uint4 w[NCH];
 #pragma unroll
-        for (uint32_t k = 0; k < 8; ++k)
+        for (uint32_t k = 0; k < NCH; ++k)
             if (k < nch) w[k] = row[k];
 #pragma unroll
-        for (uint32_t k = 0; k < 8; ++k) {
+        for (uint32_t k = 0; k < NCH; ++k) {
             if (k < nch) {
                 const uint4 qw = reinterpret_cast<const uint4*>(qp)[c0 + k];
                 qc = __builtin_amdgcn_udot4(qw.x, w[k].x, qc, false);
@@ -241,6 +243,9 @@ __device__ __forceinline__ bool first_visit(uint32_t* bits, uint32_t x) {
 //                 launch handled are skipped
 constexpr int EVAL_FLOAT = 0;
 constexpr int EVAL_INT = 1;
+// EVAL_INT for rows of at most 128 bytes (D <= 128): the row buffer of the
+// integer path is half as large
+constexpr int EVAL_INT128 = 3;
 
 template <int EF_CAP, int EVAL>
 __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
@@ -261,13 +266,13 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
     // exact integer path (see eval_fresh_int): identity SQ8, integer query
     const bool int_ok = ix.sq8_identity && (ix.dim & 15u) == 0 && ix.dim <= 256 &&
                         __ballot(!qint) == 0;
-    if (EVAL == EVAL_INT ? !int_ok : (job.skip_int && int_ok)) {
+    constexpr bool int_path = EVAL == EVAL_INT || EVAL == EVAL_INT128;
+    if (int_path ? !int_ok : (job.skip_int && int_ok)) {
         // the other launch of the pair handles this query
-        if (EVAL == EVAL_INT && job.left && lane == 0) atomicAdd(job.left, 1u);
+        if (int_path && job.left && lane == 0) atomicAdd(job.left, 1u);
         wave_sync();
         return;
     }
-    constexpr bool int_path = EVAL == EVAL_INT;
     uint32_t qq = 0;
     const uint32_t* qp = reinterpret_cast<const uint32_t*>(w.qv);
     if (int_path) {
@@ -305,7 +310,9 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
     for (int s = 0; s < S; ++s) P[s] = ~0ull;
     auto evaluate = [&](uint32_t m) -> uint64_t {
         if constexpr (EVAL == EVAL_INT)
-            return eval_fresh_int(ix, qp, qq, w.bids, m, lane);
+            return eval_fresh_int<8>(ix, qp, qq, w.bids, m, lane);
+        else if constexpr (EVAL == EVAL_INT128)
+            return eval_fresh_int<4>(ix, qp, qq, w.bids, m, lane);
         else
             return eval_fresh(ix, w.qv, pmin, pscale, w.bids, m, lane);
     };
@@ -572,6 +579,7 @@ KernelFn select_for_eval(int ef_cap, int occ) {
 }
 
 KernelFn select_kernel(const LaunchCfg& cfg) {
+    if (cfg.eval == EVAL_INT128) return select_for_eval<EVAL_INT128>(cfg.ef_cap, cfg.occupancy);
     return cfg.eval == EVAL_INT ? select_for_eval<EVAL_INT>(cfg.ef_cap, cfg.occupancy)
                                 : select_for_eval<EVAL_FLOAT>(cfg.ef_cap, cfg.occupancy);
 }

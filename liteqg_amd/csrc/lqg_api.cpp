@@ -164,11 +164,13 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
     // only the queries that were not integer valued.  Each kernel variant
     // holds only its own distance code's registers.
     const bool int_index = h->sq8_identity && h->dim % 16 == 0 && h->dim <= 256;
+    // rows of at most 128 bytes take the integer variant with half the row buffer
+    const int int_eval = h->dim <= 128 ? 3 : 1;
     if (count_visits) {
         // one N-bit bitmap per wave, for the larger grid of the two kernel
         // variants (their occupancy, hence their grid, differs)
         LaunchCfg ci = cfg, cf = cfg;
-        ci.eval = 1;
+        ci.eval = int_eval;
         cf.eval = 0;
         int groups = search_groups(h->dev(), job, cf);
         if (int_index) groups = std::max(groups, search_groups(h->dev(), job, ci));
@@ -182,7 +184,7 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
     // the integer launch left to the float launch
     LQG_HIP_TRY(launch_reset_counter(sc->counter.p, s, 3));
     if (int_index) {
-        cfg.eval = 1;
+        cfg.eval = int_eval;
         job.skip_int = 0;
         job.left = sc->counter.p + 2;
         LQG_HIP_TRY(launch_search(h->dev(), job, cfg, s));

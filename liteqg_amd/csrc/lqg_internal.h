@@ -52,7 +52,8 @@ struct LaunchCfg {
     int cache_bits;      // log2 of the per-wave visited cache (4..14)
     int groups;          // workgroups in the persistent grid (0 = fill the device)
     int occupancy;       // register cap: 8 = 64 VGPRs, 6 = 80 VGPRs, 0 = uncapped
-    int eval = 0;        // 0: SQ8 float distances; 1: exact integer distances (v_dot4)
+    int eval = 0;        // 0: SQ8 float distances; 1: exact integer distances (v_dot4);
+                         // 3: the same for rows of at most 128 bytes
 };
 
 // Device view of an index for the FastScan RaBitQ search (search_qg,
