@@ -51,18 +51,9 @@ def ground_truth(base: np.ndarray, Q: np.ndarray, k: int):
     """FP64 brute force on the device (exact for integer-valued data), ties by id."""
     import torch
 
-    B = torch.from_numpy(base).cuda().double()
-    bn = (B * B).sum(1)
-    out = np.empty((Q.shape[0], k), np.uint32)
-    for s in range(0, Q.shape[0], 512):
-        q = torch.from_numpy(Q[s:s + 512]).cuda().double()
-        d = (q * q).sum(1)[:, None] + bn[None, :] - 2.0 * (q @ B.T)
-        vals, idx = torch.topk(d, k + 16, largest=False)
-        vals, idx = vals.cpu().numpy(), idx.cpu().numpy()
-        for r in range(vals.shape[0]):
-            o = np.lexsort((idx[r], vals[r]))[:k]
-            out[s + r] = idx[r][o]
-    del B
+    from liteqg_amd.evaluate import ground_truth as gt_fp64
+
+    out = gt_fp64(base, Q, k)
     torch.cuda.empty_cache()
     return out
 
