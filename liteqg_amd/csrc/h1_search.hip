@@ -511,19 +511,56 @@ __device__ __forceinline__ void search_kernel_body(const DevIndex& ix, const Sea
     // then queries sw + atomicAdd(counter) until the batch is drained, where
     // sw is the number of waves that take part.  In counting launches a wave
     // without a bitmap takes no queries.
-    const uint32_t gw = blockIdx.x * (blockDim.x / kWave) + wid;
-    uint32_t sw = gridDim.x * (blockDim.x / kWave);
+    const uint32_t wpg = blockDim.x / kWave;
+    const uint32_t gw = blockIdx.x * wpg + wid;
+    uint32_t sw = gridDim.x * wpg;
     if (job.visit_bits) sw = min(sw, job.visit_waves);
     if (gw >= sw) return;
     uint32_t* vbits =
         job.visit_bits ? job.visit_bits + static_cast<size_t>(gw) * job.visit_words : nullptr;
-    uint32_t q = gw;
-    for (;;) {
-        if (q >= job.nq) break;
-        search_one<EF_CAP, EVAL>(ix, job, q, pmin, pscale, w, cache_bits, lane, vbits);
-        uint32_t nx = 0;
-        if (lane == 0) nx = sw + atomicAdd(job.counter, 1u);
-        q = __shfl(nx, 0);
+    if (gridDim.x < kXcds || job.visit_bits) {
+        // one queue for the grid
+        uint32_t q = gw;
+        for (;;) {
+            if (q >= job.nq) break;
+            search_one<EF_CAP, EVAL>(ix, job, q, pmin, pscale, w, cache_bits, lane, vbits);
+            uint32_t nx = 0;
+            if (lane == 0) nx = sw + atomicAdd(job.counter, 1u);
+            q = __shfl(nx, 0);
+        }
+        return;
+    }
+    // One queue per XCD.  Workgroup b runs on XCD b % 8 (round-robin
+    // dispatch); XCD x owns queries [x nq / 8, (x + 1) nq / 8): its waves take
+    // the first ones statically (wave j of the XCD takes q0 + j), the rest from
+    // the XCD's own counter (kQueueStride words apart, so the eight counters
+    // sit on separate cache lines and their atomics do not queue behind each
+    // other).  A wave whose range is drained moves on to the next XCD's
+    // counter, so the ranges drain evenly and every query is taken exactly
+    // once (each XCD has at least one workgroup, which takes its static part).
+    const uint32_t xcd = blockIdx.x % kXcds;
+    const uint32_t lw = (blockIdx.x / kXcds) * wpg + wid;
+    for (uint32_t r = 0; r < kXcds; ++r) {
+        const uint32_t x = (xcd + r) % kXcds;
+        const uint32_t nwx = ((gridDim.x + kXcds - 1u - x) / kXcds) * wpg;
+        const uint32_t q0 = static_cast<uint32_t>((static_cast<uint64_t>(x) * job.nq) / kXcds);
+        const uint32_t q1 =
+            static_cast<uint32_t>((static_cast<uint64_t>(x + 1u) * job.nq) / kXcds);
+        uint32_t* ctr = job.counter + kQueueStride * x;
+        uint32_t q;
+        if (r == 0) {
+            q = q0 + lw;
+        } else {
+            uint32_t nx = 0;
+            if (lane == 0) nx = q0 + nwx + atomicAdd(ctr, 1u);
+            q = __shfl(nx, 0);
+        }
+        while (q < q1) {
+            search_one<EF_CAP, EVAL>(ix, job, q, pmin, pscale, w, cache_bits, lane, vbits);
+            uint32_t nx = 0;
+            if (lane == 0) nx = q0 + nwx + atomicAdd(ctr, 1u);
+            q = __shfl(nx, 0);
+        }
     }
 }
 

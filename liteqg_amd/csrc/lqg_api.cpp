@@ -136,7 +136,10 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
                             s, sc);
     }
     if (nq > 0x7FFFFFFFull) return fail(LQG_ERR_ARG, "nq must be below 2^31");
-    LQG_HIP_TRY(sc->counter.reserve(3));
+    // queue words: [0, kQueueWords) integer launch, [kQueueWords, 2 kQueueWords)
+    // float launch, then the count of queries the integer launch left
+    const size_t nwords = 2 * kQueueWords + 1;
+    LQG_HIP_TRY(sc->counter.reserve(nwords));
     if (ef_class(ef_max) == 0)
         return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported on the device");
     SearchJob job;
@@ -180,19 +183,19 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
         LQG_HIP_TRY(h->d_visit.reserve(words));
         job.visit_bits = h->d_visit.p;
     }
-    // counters: [0] integer launch queue, [1] float launch queue, [2] queries
-    // the integer launch left to the float launch
-    LQG_HIP_TRY(launch_reset_counter(sc->counter.p, s, 3));
+    LQG_HIP_TRY(launch_reset_counter(sc->counter.p, s, static_cast<uint32_t>(nwords)));
+    uint32_t* left = sc->counter.p + 2 * kQueueWords;
     if (int_index) {
         cfg.eval = int_eval;
+        job.counter = sc->counter.p;
         job.skip_int = 0;
-        job.left = sc->counter.p + 2;
+        job.left = left;
         LQG_HIP_TRY(launch_search(h->dev(), job, cfg, s));
     }
     cfg.eval = 0;
     job.skip_int = int_index ? 1 : 0;
-    job.left = int_index ? sc->counter.p + 2 : nullptr;
-    job.counter = sc->counter.p + 1;
+    job.left = int_index ? left : nullptr;
+    job.counter = sc->counter.p + kQueueWords;
     LQG_HIP_TRY(launch_search(h->dev(), job, cfg, s));
     return LQG_OK;
 }

@@ -21,6 +21,12 @@ struct DevIndex {
 };
 
 // One batch of queries for the search kernel.
+// The search kernel's work queues: one counter per XCD, kQueueStride words
+// apart (a 128-byte line each).  A launch's queue block is kQueueWords words.
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kQueueStride = 32;
+constexpr uint32_t kQueueWords = kXcds * kQueueStride;
+
 struct SearchJob {
     const float* queries;   // nq x dim
     const uint32_t* ef;     // per query, or null -> ef_all
@@ -30,7 +36,7 @@ struct SearchJob {
     uint32_t* out_ids;      // nq x kstride
     float* out_d;           // nq x kstride or null
     uint32_t* out_stats;    // nq x 2 (E, V) or null
-    uint32_t* counter;      // persistent work queue head (device), zeroed before launch
+    uint32_t* counter;      // kQueueWords queue words (device), zeroed before launch
     // Exact visit counting (diagnostic launch only; null otherwise): one bitmap
     // of visit_words 32-bit words per wave of the grid.  out_stats then holds
     // (E, number of distinct nodes evaluated) instead of (E, evaluations).
