@@ -33,7 +33,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 EF_SWEEP = (20, 50, 100, 200, 500)
-FS_EF_SWEEP = (50, 100, 200, 300, 500, 700, 1000)
+FS_EF_SWEEP = (50, 100, 150, 200, 300, 500, 700, 1000)
 SERVE_RATES = (100_000.0, 1_000_000.0, 4_000_000.0)
 SERVE_LANES, SERVE_BATCH, SERVE_SLOTS = 4, 2048, 16384
 SERVE_MIX_RATES = (5_000.0, 20_000.0, 50_000.0)
