@@ -152,7 +152,7 @@ int cmd_time(int argc, char** argv) {
                 g[t]->search(&q[i * d], static_cast<uint32_t>(k), o.data());
         }
         double s = std::chrono::duration<double>(clk::now() - t0).count();
-        if (r > 0) qps_r.push_back(static_cast<double>(nq) / s);  // round 0 is warm-up
+        if (r > 0) qps_r.push_back(static_cast<double>(nq) / s);  // pass 0 is warm-up
     }
     std::sort(qps_r.begin(), qps_r.end());
     std::printf(

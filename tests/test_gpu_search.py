@@ -137,6 +137,32 @@ def test_uneven_ef_and_k_in_one_launch(oracle_mod, tmp_path):
         assert (ids[i, kk:] == 0xFFFFFFFF).all()
 
 
+@pytest.mark.parametrize("wpg", [1, 4])
+def test_batch_sizes_around_the_xcd_split(oracle_mod, tmp_path, wpg):
+    """Batches from 1 to 257 queries: below 8 workgroups one queue serves the
+    grid, from 8 up each XCD owns nq/8 queries and drains the others' ranges
+    when its own is empty.  Every query is searched once, with the oracle's
+    results, on the integer and on the float path."""
+    for kind, d in (("int", 64), ("uniform", 48)):
+        arr, base = make_random_index(4000, d, 32, seed=d, kind=kind, knn=True)
+        ix = device_from_arrays(arr, tmp_path, tag=kind)
+        ix.set_launch(0, waves_per_group=wpg, cache_bits=10)
+        rng = np.random.default_rng(d)
+        noise = 4.0 if kind == "int" else 0.05
+        Qall = base[rng.integers(0, 4000, 257)] + rng.normal(0, noise, size=(257, d))
+        if kind == "int":
+            Qall = np.clip(np.rint(Qall), 0, 255)
+        Qall = Qall.astype(np.float32)
+        for nq in (1, 7, 8, 9, 33, 64, 257):
+            Q = Qall[:nq]
+            ids, dis, st = ix.search_batch(Q, 10, ef=40, order="nearest", return_distances=True,
+                                           return_stats=True)
+            exp, exp_d, est = oracle_batch(oracle_mod, arr, Q, 40, 10, "nearest")
+            assert_same(ids, dis, exp, exp_d)
+            assert st["queries"] == nq and st["expansions"] == int(est[:, 0].sum()), (kind, nq)
+        ix.reset_launch(0)
+
+
 def test_repeated_calls_identical(tmp_path):
     arr, base = make_random_index(6000, 128, 32, seed=9, kind="int", knn=True)
     ix = device_from_arrays(arr, tmp_path)
