@@ -216,6 +216,64 @@ def serving_mix_leg(args):
     return out
 
 
+def configs_leg(args):
+    """Two more workloads at full size, SQ8 path, order=nearest, graphs built on
+    the device (3 iterations), FP64 ground truth, 10 timed launches per ef:
+      100,000 x 64 i.i.d. U[0,1) (the reference README's example), degree 32,
+        build EF 200, 1,000 queries, ef {50, 100, 200}, k 10; the plain path
+        and the reference's CPU search at ef 100 beside it;
+      10,000,000 x 96 unit-norm, 4,096 centres N(0, I), sigma 0.3 before
+        normalisation, degree 32, build EF 300, 10,000 queries, ef {50, 200},
+        k 10; the batch also replayed as a HIP graph."""
+    import torch
+
+    from liteqg_amd import datasets
+    from liteqg_amd.symphonyqg import Index
+
+    out = {}
+    for name, n, d, efb, nq, kind, kw, efs in (
+            ("uniform_100k_d64", 100_000, 64, 200, 1_000, "uniform", {}, (50, 100, 200)),
+            ("unitmix_10m_d96", 10_000_000, 96, 300, 10_000, "unit_mix",
+             dict(centres=4096, sigma=0.3, centre_law="normal"), (50, 200))):
+        t = time.perf_counter()
+        base, Q = datasets.make(kind, n, d, nq, seed=args.seed, **kw)
+        ix = Index("QG", "L2", n, d, 32)
+        ix.build_index(base, efb, num_iter=3)
+        build_s = time.perf_counter() - t
+        gt = ground_truth(base, Q, 10)
+        db = DeviceBatch(ix, Q, 10)
+        rows = []
+        for ef in efs:
+            db.launch(ef)
+            torch.cuda.synchronize()
+            rec = recall_at_k(db.results(), gt, 10)
+            ms, _ = time_launches(db, ef, 10, 2)
+            row = {"ef": ef, "recall": round(rec, 4), "qps": round(nq / ms * 1e3, 1),
+                   "ms_per_step": round(ms, 4)}
+            if name.startswith("unitmix"):
+                row["graph_replay_ms"] = round(graph_leg(ix, db, ef, args), 4)
+            rows.append(row)
+        res = {"data_and_build_s": round(build_s, 1), "queries": nq, "ef_sweep": rows}
+        if name.startswith("uniform"):
+            ix.set_ef(100)
+            ix.set_order("nearest")
+            t = time.perf_counter()
+            for q in Q[:300]:
+                ix.search(q, 10)
+            res["plain_qps_ef100"] = round(300 / (time.perf_counter() - t), 1)
+            if not args.no_cpu:
+                with tempfile.TemporaryDirectory() as tmp:
+                    p = os.path.join(tmp, "c.index")
+                    ix.save(p)
+                    res["cpu_reference_ef100"] = cpu_baseline(p, n, d, 32, Q, 100, 10,
+                                                              min(16, os.cpu_count() or 1))
+        out[name] = res
+        log(f"config {name}: {res}")
+        del ix, db, base
+        torch.cuda.empty_cache()
+    return out
+
+
 def fastscan_leg(ix, db, Q, gt, k, args):
     """The FastScan RaBitQ path (search_qg) on the same index and queries:
     device sidecar build, ef sweep to recall@10 >= 0.95, batched timing, and
@@ -350,6 +408,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fastscan", action="store_true", help="skip the FastScan RaBitQ leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving legs")
+    ap.add_argument("--all-configs", action="store_true",
+                    help="also measure the 100k x 64 and 10M x 96 workloads (about a minute)")
     ap.add_argument("--serve-n", type=int, default=1_000_000,
                     help="base vectors of the 768-d serving workload")
     ap.add_argument("--serve-nq", type=int, default=10_000,
@@ -473,6 +533,8 @@ def main():
 
     serving_mix = None if args.no_serve else serving_mix_leg(args)
 
+    configs = configs_leg(args) if args.all_configs else None
+
     cpu = None
     if not args.no_cpu:
         with tempfile.TemporaryDirectory() as tmp:
@@ -520,6 +582,8 @@ def main():
         line["serving"] = serving
     if serving_mix:
         line["serving_mix"] = serving_mix
+    if configs:
+        line["other_workloads"] = configs
     if fastscan:
         line["fastscan"] = fastscan
     if cpu:
