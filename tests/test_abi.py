@@ -215,3 +215,27 @@ def test_fastscan_graph_and_serving_validation():
     assert L.lqg_serve_start(ix._h, C.byref(_lib.ServeConfig(2, 8, 8, 64, 10, 1)),
                              C.byref(s)) == 2
     assert L.lqg_serve_stop(None) == 0
+
+
+def test_header_is_plain_c(tmp_path):
+    """include/liteqg.h compiles as C99 (pedantic, warnings as errors) and as C++,
+    so C, cgo-style and ctypes hosts can use it as is."""
+    import shutil
+    import subprocess
+
+    src = tmp_path / "use.c"
+    src.write_text('#include "liteqg.h"\n'
+                   "int f(lqg_index* h) {\n"
+                   "    lqg_launch c; lqg_mode m; lqg_rotation r; lqg_serve_config s;\n"
+                   "    (void)s; lqg_get_mode(h, &m); lqg_h2_get_rotation(h, &r);\n"
+                   "    return lqg_get_launch(h, 50, &c) + (int)LQG_ORDER_NEAREST;\n"
+                   "}\n")
+    inc = os.path.join(ROOT, "include")
+    for cc, std in (("gcc", "-std=c99"), ("g++", "-std=c++17")):
+        if not shutil.which(cc):
+            pytest.skip(f"{cc} not available")
+        args = [cc, std, "-Wall", "-Wextra", "-pedantic", "-Werror", "-fsyntax-only", "-I", inc]
+        if cc == "g++":
+            args += ["-x", "c++"]
+        r = subprocess.run(args + [str(src)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
