@@ -489,15 +489,23 @@ __device__ __forceinline__ void search_kernel_body(const DevIndex& ix, const Sea
     extern __shared__ __align__(16) unsigned char lds[];
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wid = threadIdx.x / kWave;
-    float* pmin = reinterpret_cast<float*>(lds);
-    float* pscale = pmin + dim_pad;
-    for (uint32_t i = threadIdx.x; i < ix.dim; i += blockDim.x) {
-        pmin[i] = ix.sq8_min[i];
-        pscale[i] = ix.sq8_scale[i];
+    // only the float path decodes with the SQ8 parameters; the integer
+    // variants' workgroups leave that region out of their LDS
+    constexpr bool kParams = EVAL == EVAL_FLOAT;
+    float* pmin = nullptr;
+    float* pscale = nullptr;
+    if constexpr (kParams) {
+        pmin = reinterpret_cast<float*>(lds);
+        pscale = pmin + dim_pad;
+        for (uint32_t i = threadIdx.x; i < ix.dim; i += blockDim.x) {
+            pmin[i] = ix.sq8_min[i];
+            pscale[i] = ix.sq8_scale[i];
+        }
+        __syncthreads();
     }
-    __syncthreads();
 
-    unsigned char* wb = lds + 8u * dim_pad + static_cast<size_t>(wid) * wave_bytes;
+    unsigned char* wb =
+        lds + (kParams ? 8u * dim_pad : 0u) + static_cast<size_t>(wid) * wave_bytes;
     WaveLds w;
     w.qv = reinterpret_cast<float*>(wb);
     w.stage = reinterpret_cast<uint64_t*>(wb + 4u * dim_pad);
@@ -633,7 +641,7 @@ void geometry(const DevIndex& ix, const LaunchCfg& cfg, uint32_t& dim_pad, uint3
 size_t search_lds_bytes(const DevIndex& ix, const LaunchCfg& cfg) {
     uint32_t dim_pad, wave_bytes;
     geometry(ix, cfg, dim_pad, wave_bytes);
-    return 8u * static_cast<size_t>(dim_pad) +
+    return (cfg.eval == EVAL_FLOAT ? 8u * static_cast<size_t>(dim_pad) : 0u) +
            static_cast<size_t>(cfg.waves_per_group) * wave_bytes;
 }
 
