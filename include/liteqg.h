@@ -207,7 +207,7 @@ lqg_status lqg_serve_replay(lqg_server* server, const float* queries, uint64_t n
                             uint64_t* latency_ns, uint32_t* results);
 
 /* Launch settings of the search kernel for one pool-capacity class
- * (ef <= 64, 128, 256, 512, 1024).  No counterpart in the reference (CPU
+ * (ef <= 64, 128, 192, 256, 512, 1024).  No counterpart in the reference (CPU
  * code has no launch geometry). */
 typedef struct {
     int32_t waves_per_group; /* wavefronts (queries in flight) per workgroup, 1..4      */

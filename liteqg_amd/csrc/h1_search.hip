@@ -616,6 +616,7 @@ KernelFn select_for_eval(int ef_cap, int occ) {
     switch (ef_cap) {
         case 64: return kernel_for<64, EVAL>(occ);
         case 128: return kernel_for<128, EVAL>(occ);
+        case 192: return kernel_for<192, EVAL>(occ);
         case 256: return kernel_for<256, EVAL>(occ);
         case 512: return kernel_for<512, EVAL>(occ);
         case 1024: return kernel_for<1024, EVAL>(occ);

@@ -1034,6 +1034,7 @@ SearchFn select_search(int ef_cap, int cw, int occ) {
     switch (ef_cap) {
         case 64: return search_for_cw<64>(cw, occ);
         case 128: return search_for_cw<128>(cw, occ);
+        case 192: return search_for_cw<192>(cw, occ);
         case 256: return search_for_cw<256>(cw, occ);
         case 512: return search_for_cw<512>(cw, occ);
         case 1024: return search_for_cw<1024>(cw, occ);

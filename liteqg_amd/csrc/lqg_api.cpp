@@ -33,10 +33,14 @@ lqg_status hip_fail(hipError_t e, const char* what) {
 const char* last_error() { return g_err.c_str(); }
 
 int ef_class(uint32_t ef) {
-    const int classes[] = {64, 128, 256, 512, 1024};
-    for (int c : classes)
-        if (ef <= static_cast<uint32_t>(c)) return c;
+    for (int slot = 0; slot < kClasses; ++slot)
+        if (ef <= static_cast<uint32_t>(class_cap(slot))) return class_cap(slot);
     return 0;
+}
+
+int class_cap(int slot) {
+    static const int caps[kClasses] = {64, 128, 192, 256, 512, 1024};
+    return caps[slot];
 }
 
 lqg_status ensure_device(lqg_index* h) {
@@ -106,13 +110,9 @@ lqg_status ensure_device(lqg_index* h) {
 }
 
 int class_slot(int ef_cap) {
-    switch (ef_cap) {
-        case 64: return 0;
-        case 128: return 1;
-        case 256: return 2;
-        case 512: return 3;
-        default: return 4;
-    }
+    for (int slot = 0; slot < kClasses - 1; ++slot)
+        if (ef_cap <= class_cap(slot)) return slot;
+    return kClasses - 1;
 }
 
 LaunchCfg default_launch(int ef_cap) {
@@ -496,8 +496,8 @@ lqg_status lqg_set_launch(lqg_index* h, uint32_t ef, const lqg_launch* cfg) {
         return fail(LQG_ERR_UNSUPPORTED,
                     "FastScan occupancy caps need at most 4 code words per lane (P / lanes per "
                     "neighbour <= 128 bits)");
-    for (int slot = 0; slot < 5; ++slot) {
-        const int cap = 64 << slot;
+    for (int slot = 0; slot < lqg::kClasses; ++slot) {
+        const int cap = lqg::class_cap(slot);
         if (ef != 0 && ef_class(ef) != cap) continue;
         if (fs) {
             if (!cfg) {

@@ -58,11 +58,16 @@ struct Scratch {
     }
 };
 
+// Pool-capacity classes: 64, 128, 192, 256, 512, 1024 (a kernel variant each;
+// 192 = three pool slots per lane, so ef up to 192 does not pay for four).
+constexpr int kClasses = 6;
+
 // Smallest pool-capacity class that holds ef, 0 if none does.
 int ef_class(uint32_t ef);
 
-// Index 0..4 of a capacity class (64..1024).
+// Index 0..kClasses-1 of a capacity class, and the capacity of an index.
 int class_slot(int ef_cap);
+int class_cap(int slot);
 
 // Launch settings used when none were set for a class.
 LaunchCfg default_launch(int ef_cap);
@@ -97,9 +102,9 @@ struct lqg_index {
     lqg::DevBuf<float> d_dist;
     lqg::DevBuf<uint32_t> d_visit;  // exact visit counting bitmaps
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // launch settings per pool-capacity class (64, 128, 256, 512, 1024)
-    lqg::LaunchCfg launch[5];
-    bool launch_set[5] = {false, false, false, false, false};
+    // launch settings per pool-capacity class (64, 128, 192, 256, 512, 1024)
+    lqg::LaunchCfg launch[lqg::kClasses];
+    bool launch_set[lqg::kClasses] = {};
 
     // FastScan RaBitQ search (search_qg, qg.hpp:252-287): node records on the
     // device (raw | codes | factors | ids) and the rotation signs
@@ -109,8 +114,8 @@ struct lqg_index {
     std::vector<float> h2_signs;
     lqg::DevBuf<uint8_t> d_h2rec;
     lqg::DevBuf<float> d_h2signs;
-    lqg::H2Cfg h2_launch[5];
-    bool h2_launch_set[5] = {false, false, false, false, false};
+    lqg::H2Cfg h2_launch[lqg::kClasses];
+    bool h2_launch_set[lqg::kClasses] = {};
     lqg_rotation h2_rotation = LQG_ROTATION_FHT;
 
     lqg::H2Dev h2dev() const {

@@ -53,7 +53,7 @@ struct SearchJob {
 
 // Launch geometry chosen by the host (autotuner or defaults).
 struct LaunchCfg {
-    int ef_cap;          // pool capacity class: 64, 128, 256, 512 or 1024
+    int ef_cap;          // pool capacity class: 64, 128, 192, 256, 512 or 1024
     int waves_per_group; // waves per workgroup
     int cache_bits;      // log2 of the per-wave visited cache (4..14)
     int groups;          // workgroups in the persistent grid (0 = fill the device)
@@ -93,7 +93,7 @@ struct H2Job {
 };
 
 struct H2Cfg {
-    int ef_cap;          // 64 .. 1024
+    int ef_cap;          // pool capacity class, as for LaunchCfg
     int waves_per_group; // 1..4
     int groups;          // persistent grid size (0 = fill the device)
     int hash_bits;       // log2 entries of the per-wave LDS visited table

@@ -198,6 +198,12 @@ def test_fastscan_graph_and_serving_validation():
     ix.set_launch(0, waves_per_group=2, cache_bits=10, occupancy=6)
     assert ix.get_launch(64)["occupancy"] == 6
     ix.reset_launch(0)
+    # settings are per pool-capacity class; ef 129..192 is one class
+    ix.set_launch(150, waves_per_group=3, cache_bits=7, occupancy=0)
+    assert ix.get_launch(129)["cache_bits"] == 7 and ix.get_launch(192)["cache_bits"] == 7
+    assert ix.get_launch(128)["cache_bits"] != 7 and ix.get_launch(193)["cache_bits"] != 7
+    ix.reset_launch(0)
+    assert ix.get_launch(150)["cache_bits"] != 7
     # graph capture: null output, zero queries, unknown order
     g = C.c_void_p()
     assert L.lqg_graph_create(ix._h, None, 1, None, None, 10, 10, 1, None, None, None,

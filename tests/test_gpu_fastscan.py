@@ -147,7 +147,7 @@ def test_search_matches_oracle(oracle_mod, tmp_path, shape):
     ix.set_mode("fastscan")
     rng = np.random.default_rng(d)
     Q = (arr.raw[rng.integers(0, n, 40)] + rng.normal(0, 0.05, (40, d))).astype(np.float32)
-    for ef, k in ((16, 10), (64, 10), (200, 20), (700, 10)):
+    for ef, k in ((16, 10), (64, 10), (150, 10), (192, 20), (200, 20), (700, 10)):
         ids, dis, st = ix.search_batch(Q, k, ef=ef, return_distances=True, return_stats=True)
         e_ids, e_dis, e_st = _oracle_h2(O, arr, ix, Q, ef, k)
         assert np.array_equal(ids, e_ids), f"ef={ef}"
@@ -180,7 +180,7 @@ def test_occupancy_variants_match_oracle(oracle_mod, tmp_path, shape):
         assert e.value.status == 6
         return
     for occ in (5, 6):
-        for ef, k, bits in ((64, 10, 10), (200, 20, 10), (256, 10, 6)):
+        for ef, k, bits in ((64, 10, 10), (150, 10, 10), (200, 20, 10), (256, 10, 6)):
             ix.set_launch(0, waves_per_group=4, cache_bits=bits, occupancy=occ)
             assert ix.get_launch(ef)["occupancy"] == occ
             ids, dis, st = ix.search_batch(Q, k, ef=ef, return_distances=True, return_stats=True)

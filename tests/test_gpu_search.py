@@ -86,6 +86,9 @@ CASES = [
     # n, d, deg, ef, k, kind, knn, dup_rate
     (20000, 128, 32, 100, 10, "int", True, 0.0),
     (20000, 64, 32, 200, 10, "uniform", True, 0.0),
+    (20000, 128, 32, 150, 10, "int", True, 0.0),      # 192 class, integer kernel
+    (5000, 64, 32, 192, 10, "uniform", True, 0.0),    # 192 class at its boundary
+    (5000, 64, 32, 129, 10, "uniform", True, 0.0),    # 192 class just above 128
     (5000, 96, 64, 64, 10, "uniform", True, 0.0),
     (3000, 80, 32, 50, 10, "uniform", True, 0.0),      # D % 16 != 0: FMA tail
     (3000, 100, 32, 50, 10, "uniform", True, 0.0),     # D % 4 != 0: byte loads
