@@ -12,7 +12,11 @@
 //     estimator that reads codes LiteQG's rows do not hold, qg.hpp:348-369);
 //   * reverse edges are gathered for all nodes and pruned once per node
 //     instead of being inserted one at a time under per-node locks;
-//   * random draws come from a seeded counter hash, so builds are repeatable.
+//   * random draws come from a seeded counter hash, and reverse edges are
+//     ordered by a stable sort, so builds are repeatable: the same data and
+//     parameters give the same graph, row for row.  (With atomics ordering
+//     them, a node with more reverse edges than its candidate buffer holds
+//     would keep whichever arrived first.)
 //
 // One wavefront processes one node.  The sequential prune rules (keep a
 // candidate unless an already kept neighbour dominates it) are evaluated 64
@@ -290,7 +294,6 @@ struct BuildArgs {
     uint32_t* pr_n;            // N
     uint32_t pr_cap;
     const uint32_t* in_off;    // N + 1
-    uint32_t* in_fill;         // N
     uint32_t* in_ids;          // N x deg total
     uint32_t* counter;
     uint64_t seed;
@@ -419,15 +422,22 @@ __global__ void count_reverse(BuildArgs a, uint32_t* in_cnt) {
     atomicAdd(&in_cnt[a.kept_ids[e]], 1u);
 }
 
-// Reverse edges, pass 2: scatter the sources into CSR rows.
-__global__ void scatter_reverse(BuildArgs a) {
+// Reverse edges, pass 2: one (target, source) pair per edge slot, the key N
+// for empty slots.  Slots are in source order, so a stable sort by target
+// gives every CSR row its sources in ascending order, the N keys last.
+__global__ void emit_reverse(BuildArgs a, uint32_t* keys, uint32_t* vals) {
     const size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= static_cast<size_t>(a.n) * a.deg) return;
     const uint32_t i = static_cast<uint32_t>(e / a.deg), s = static_cast<uint32_t>(e % a.deg);
-    if (s >= a.kept_n[i]) return;
-    const uint32_t j = a.kept_ids[e];
-    const uint32_t pos = a.in_off[j] + atomicAdd(&a.in_fill[j], 1u);
-    a.in_ids[pos] = i;
+    keys[e] = s < a.kept_n[i] ? a.kept_ids[e] : a.n;
+    vals[e] = i;
+}
+
+// Key bits for targets 0..n (n = the empty-slot key).
+uint32_t reverse_key_bits(uint32_t n) {
+    uint32_t bits = 1;
+    while (bits < 32 && (n >> bits) != 0) ++bits;
+    return bits;
 }
 
 // Phase B: own kept list plus reverse edges, pruned again (add_reverse_edges,
@@ -567,6 +577,20 @@ size_t scan_temp_bytes(uint32_t n) {
     return bytes;
 }
 
+size_t build_temp_bytes(uint32_t n, uint32_t deg) {
+    size_t sort_bytes = 0;
+    const int edges = static_cast<int>(static_cast<size_t>(n) * deg);
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, static_cast<const uint32_t*>(nullptr),
+                                           static_cast<uint32_t*>(nullptr),
+                                           static_cast<const uint32_t*>(nullptr),
+                                           static_cast<uint32_t*>(nullptr), edges, 0,
+                                           static_cast<int>(reverse_key_bits(n))) != hipSuccess)
+        return 0;
+    const size_t scan_bytes = scan_temp_bytes(n + 1);
+    if (scan_bytes == 0) return 0;
+    return sort_bytes > scan_bytes ? sort_bytes : scan_bytes;
+}
+
 hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, void* tmp,
                               size_t tmp_bytes, hipStream_t s) {
     return hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, static_cast<int>(n), s);
@@ -592,7 +616,6 @@ hipError_t build_run_iteration(const BuildBuffers& b, const BuildParams& p, hipS
     a.pr_n = b.pr_n;
     a.pr_cap = p.pr_cap;
     a.in_off = b.in_off;
-    a.in_fill = b.in_fill;
     a.in_ids = b.in_ids;
     a.counter = b.counter;
     a.seed = p.seed;
@@ -609,9 +632,14 @@ hipError_t build_run_iteration(const BuildBuffers& b, const BuildParams& p, hipS
     if ((e = exclusive_scan_u32(b.in_cnt, b.in_off, p.n + 1, b.scan_tmp, b.scan_tmp_bytes, s)) !=
         hipSuccess)
         return e;
-    if ((e = hipMemsetAsync(b.in_fill, 0, sizeof(uint32_t) * p.n, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(scatter_reverse, dim3(grid), dim3(blk), 0, s, a);
+    hipLaunchKernelGGL(emit_reverse, dim3(grid), dim3(blk), 0, s, a, b.rev_keys, b.rev_vals);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    size_t sort_bytes = b.scan_tmp_bytes;
+    if ((e = hipcub::DeviceRadixSort::SortPairs(b.scan_tmp, sort_bytes, b.rev_keys, b.rev_keys_out,
+                                                b.rev_vals, b.in_ids, static_cast<int>(edges), 0,
+                                                static_cast<int>(reverse_key_bits(p.n)), s)) !=
+        hipSuccess)
+        return e;
     if ((e = launch_reset_counter(b.counter, s)) != hipSuccess) return e;
     return launch_node_kernel(merge_reverse, a, s);
 }

@@ -172,9 +172,13 @@ lqg_status build_index(lqg_index* h, const float* data, uint32_t ef_build, uint3
     p.seed = kSeed;
 
     DevBuf<float> raw, kept_d, pr_d;
-    DevBuf<uint32_t> cand, kept_ids, kept_n, pr_ids, pr_n, in_cnt, in_off, in_fill, in_ids, ctr;
+    DevBuf<uint32_t> cand, kept_ids, kept_n, pr_ids, pr_n, in_cnt, in_off, in_ids, rev_k, rev_ko,
+        rev_v, ctr;
     DevBuf<unsigned char> tmp;
-    const size_t tmp_bytes = scan_temp_bytes(p.n + 1);
+    if (n * g > static_cast<size_t>(INT32_MAX))
+        return fail(LQG_ERR_UNSUPPORTED, "build: N x degree above 2^31 edge slots");
+    const size_t tmp_bytes = build_temp_bytes(p.n, p.deg);
+    if (tmp_bytes == 0) return fail(LQG_ERR_DEVICE, "builder scratch size query failed");
     LQG_HIP_TRY(raw.reserve(n * d));
     LQG_HIP_TRY(cand.reserve(n * p.ncand));
     LQG_HIP_TRY(kept_ids.reserve(n * g));
@@ -185,8 +189,10 @@ lqg_status build_index(lqg_index* h, const float* data, uint32_t ef_build, uint3
     LQG_HIP_TRY(pr_n.reserve(n));
     LQG_HIP_TRY(in_cnt.reserve(n + 1));
     LQG_HIP_TRY(in_off.reserve(n + 1));
-    LQG_HIP_TRY(in_fill.reserve(n));
     LQG_HIP_TRY(in_ids.reserve(n * g));
+    LQG_HIP_TRY(rev_k.reserve(n * g));
+    LQG_HIP_TRY(rev_ko.reserve(n * g));
+    LQG_HIP_TRY(rev_v.reserve(n * g));
     LQG_HIP_TRY(ctr.reserve(1));
     LQG_HIP_TRY(tmp.reserve(tmp_bytes));
     LQG_HIP_TRY(hipMemcpy(raw.p, h->raw.data(), n * d * 4, hipMemcpyHostToDevice));
@@ -206,8 +212,10 @@ lqg_status build_index(lqg_index* h, const float* data, uint32_t ef_build, uint3
     b.pr_n = pr_n.p;
     b.in_cnt = in_cnt.p;
     b.in_off = in_off.p;
-    b.in_fill = in_fill.p;
     b.in_ids = in_ids.p;
+    b.rev_keys = rev_k.p;
+    b.rev_keys_out = rev_ko.p;
+    b.rev_vals = rev_v.p;
     b.counter = ctr.p;
     b.scan_tmp = tmp.p;
     b.scan_tmp_bytes = tmp_bytes;
@@ -239,8 +247,10 @@ lqg_status build_index(lqg_index* h, const float* data, uint32_t ef_build, uint3
     pr_n.release();
     in_cnt.release();
     in_off.release();
-    in_fill.release();
     in_ids.release();
+    rev_k.release();
+    rev_ko.release();
+    rev_v.release();
     ctr.release();
     tmp.release();
     if (rs != LQG_OK) {

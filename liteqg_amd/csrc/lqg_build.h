@@ -29,15 +29,21 @@ struct BuildBuffers {
     uint32_t* pr_n;      // N
     uint32_t* in_cnt;    // N + 1
     uint32_t* in_off;    // N + 1
-    uint32_t* in_fill;   // N
-    uint32_t* in_ids;    // N x deg
+    uint32_t* in_ids;    // N x deg: reverse-edge sources, CSR rows by target
+    uint32_t* rev_keys;      // N x deg: edge targets (sort input)
+    uint32_t* rev_keys_out;  // N x deg: sorted targets
+    uint32_t* rev_vals;      // N x deg: edge sources (sort input)
     uint32_t* counter;   // 1
-    void* scan_tmp;
+    void* scan_tmp;      // build_temp_bytes(N, deg) bytes: scan and sort scratch
     size_t scan_tmp_bytes;
 };
 
 // Bytes of temporary storage exclusive_scan_u32 needs for n elements.
 size_t scan_temp_bytes(uint32_t n);
+
+// Bytes of temporary storage one builder iteration needs (the larger of the
+// reverse-edge scan and sort), 0 on error.
+size_t build_temp_bytes(uint32_t n, uint32_t deg);
 
 // out[i] = sum(in[0..i)), i < n.
 hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, void* tmp,
