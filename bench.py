@@ -13,6 +13,11 @@ stream the kernel runs on, inputs already in HBM).  The plain path (one
 query per Index.search call, host buffers) is timed beside it for the
 batched/plain ratio and p50/p99 latency.
 
+``steady_state`` (beside ``value``, not in it): the same 10,000-query batch
+replayed back to back from four HIP graphs on four streams, so each batch's
+tail overlaps the next batch's start; ids are checked equal to the single
+launch's.  It shows the kernel's rate without the tail of a lone batch.
+
 Usage:  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
 from __future__ import annotations
@@ -33,7 +38,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 EF_SWEEP = (20, 50, 100, 200, 500)
-FS_EF_SWEEP = (50, 100, 150, 200, 300, 500, 700, 1000)
+FS_EF_SWEEP = (50, 100, 128, 150, 200, 300, 500, 700, 1000)
 SERVE_RATES = (100_000.0, 1_000_000.0, 4_000_000.0)
 SERVE_LANES, SERVE_BATCH, SERVE_SLOTS = 4, 2048, 16384
 SERVE_MIX_RATES = (5_000.0, 20_000.0, 50_000.0)
@@ -137,6 +142,57 @@ def graph_leg(ix, db, ef, args):
     torch.cuda.synchronize()
     g.close()
     return t0.elapsed_time(t1) / args.steps
+
+
+def steady_state_leg(ix, db, ef, args, depth=4):
+    """Batches back to back: `depth` HIP-graph replays of the same batch
+    launch, each with its own result buffer and work-queue scratch, taken in
+    turn on `depth` streams, so the tail of one batch (its last queries, when
+    most waves have no query left) overlaps the start of the next.  K steps
+    of one batch each are timed between events on the first stream; every
+    replay's ids must equal the single launch's.  HIP maps streams onto its
+    few hardware queues (4 per process here), and two streams that share one
+    run their launches one after the other; with four streams at least two
+    queues take part."""
+    import torch
+
+    from liteqg_amd.serving import BatchGraph
+
+    db.launch(ef)
+    torch.cuda.synchronize()
+    ref = db.results().copy()
+    outs = [torch.full((db.nq, db.k), -1, dtype=torch.int32, device="cuda")
+            for _ in range(depth)]
+    graphs = [BatchGraph(ix, db.q.data_ptr(), db.nq, o.data_ptr(), db.k, ef) for o in outs]
+    streams = [torch.cuda.Stream() for _ in range(depth)]
+
+    def run(steps):
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(streams[0])
+        for s in streams[1:]:
+            s.wait_event(t0)
+        for i in range(steps):
+            graphs[i % depth].launch(streams[i % depth].cuda_stream)
+        for s in streams[1:]:
+            e = torch.cuda.Event()
+            e.record(s)
+            streams[0].wait_event(e)
+        t1.record(streams[0])
+        torch.cuda.synchronize()
+        return t0.elapsed_time(t1) / steps
+
+    try:
+        run(max(args.warmup, depth))
+        ms = run(args.steps)
+        same = all(np.array_equal(o.cpu().numpy().view(np.uint32), ref) for o in outs)
+    finally:
+        for g in graphs:
+            g.close()
+    if not same:
+        raise RuntimeError("a back-to-back batch returned ids other than the single launch's")
+    return {"streams": depth, "ms_per_batch": round(ms, 4),
+            "qps": round(db.nq / (ms / 1e3), 1), "ids_equal_single_launch": same}
 
 
 def serving_leg(ix, Q, ef, k, args):
@@ -312,6 +368,11 @@ def fastscan_leg(ix, db, Q, gt, k, args):
     rec_bytes = 4 * d + deg * p // 8 + 16 * deg
     alg = float(np.sum(4 * d + E * rec_bytes + 4 * k))
     achieved = alg / (ms_step / 1e3) / 1e9
+    steady = steady_state_leg(ix, db, chosen, args)
+    steady_gbs = alg / (steady["ms_per_batch"] / 1e3) / 1e9
+    steady.update({"algorithmic_GBps": round(steady_gbs, 1),
+                   "frac_of_peak": round(steady_gbs / HBM_PEAK_GBS, 4)})
+    log(f"fastscan batches back to back: {steady}")
     # the matrix-core rotation: its error against FP64 (the FHT's beside it),
     # then the same search with GEMM-rotated queries
     import scipy.linalg
@@ -341,7 +402,7 @@ def fastscan_leg(ix, db, Q, gt, k, args):
     return {"path": "FastScan RaBitQ search_qg (1-bit codes, exact re-rank)", "ef": chosen,
             "recall_at_10": rec, "qps": round(qps, 1), "ms_per_step": round(ms_step, 4),
             "ef_sweep": sweep, "sidecar_build_s": round(side_s, 2), "launch": tuned,
-            "gemm_rotation": gemm,
+            "gemm_rotation": gemm, "steady_state": steady,
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "traffic": read_traffic(TRAFFIC_FILE_FS),
@@ -501,6 +562,8 @@ def main():
     # the same launch captured into a HIP graph and replayed
     graph_ms = graph_leg(ix, db, chosen, args)
     log(f"graph replay: {graph_ms:.3f} ms/step")
+    steady = steady_state_leg(ix, db, chosen, args)
+    log(f"batches back to back: {steady}")
 
     # plain path: one query per Index.search call
     ix.set_ef(chosen)
@@ -578,6 +641,11 @@ def main():
     }
     line["graph_replay"] = {"ms_per_step": round(graph_ms, 4),
                             "qps": round(args.nq / (graph_ms / 1e3), 1)}
+    # the same algorithmic bytes per batch over the back-to-back batch time
+    steady_gbs = alg_bytes / (steady["ms_per_batch"] / 1e3) / 1e9
+    steady.update({"algorithmic_GBps": round(steady_gbs, 1),
+                   "frac_of_peak": round(steady_gbs / HBM_PEAK_GBS, 4)})
+    line["steady_state"] = steady
     if serving:
         line["serving"] = serving
     if serving_mix:

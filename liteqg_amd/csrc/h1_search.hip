@@ -530,6 +530,7 @@ __device__ __forceinline__ void search_kernel_body(const DevIndex& ix, const Sea
         // one queue for the grid
         uint32_t q = gw;
         for (;;) {
+            q = uniform(q);
             if (q >= job.nq) break;
             search_one<EF_CAP, EVAL>(ix, job, q, pmin, pscale, w, cache_bits, lane, vbits);
             uint32_t nx = 0;
@@ -557,17 +558,17 @@ __device__ __forceinline__ void search_kernel_body(const DevIndex& ix, const Sea
         uint32_t* ctr = job.counter + kQueueStride * x;
         uint32_t q;
         if (r == 0) {
-            q = q0 + lw;
+            q = uniform(q0 + lw);
         } else {
             uint32_t nx = 0;
             if (lane == 0) nx = q0 + nwx + atomicAdd(ctr, 1u);
-            q = __shfl(nx, 0);
+            q = uniform(__shfl(nx, 0));
         }
         while (q < q1) {
             search_one<EF_CAP, EVAL>(ix, job, q, pmin, pscale, w, cache_bits, lane, vbits);
             uint32_t nx = 0;
             if (lane == 0) nx = q0 + nwx + atomicAdd(ctr, 1u);
-            q = __shfl(nx, 0);
+            q = uniform(__shfl(nx, 0));
         }
     }
 }

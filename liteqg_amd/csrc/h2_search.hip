@@ -746,6 +746,7 @@ __device__ __forceinline__ void h2_search_body(const H2Dev& ix, const H2Job& job
     const uint32_t sw = gridDim.x * (blockDim.x / kWave);
     uint32_t q = gwave;
     for (;;) {
+        q = uniform(q);
         if (q >= job.nq) break;
         h2_search_one<EF_CAP, CW>(ix, job, q, w, hash_bits, gtab, lane);
         uint32_t nx = 0;

@@ -32,4 +32,12 @@ __device__ __forceinline__ uint32_t readlane32(uint32_t v, int l) {
     return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), l));
 }
 
+// A value the caller knows to be equal on every lane (a query index taken by
+// lane 0 and broadcast, say), moved to a scalar register.  The compiler then
+// keeps what is derived from it (per-query ef and k, pool sizes, loop bounds)
+// scalar too, instead of in vector registers under exec masks.
+__device__ __forceinline__ uint32_t uniform(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+}
+
 }  // namespace lqg

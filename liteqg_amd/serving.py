@@ -31,7 +31,8 @@ class Server:
         cfg = ServeConfig(int(lanes), int(max_batch), int(slots or 4 * lanes * max_batch),
                           int(ef_max), int(kmax), ORDER[order])
         self._s = C.c_void_p()
-        check(lib().lqg_serve_start(index._h, C.byref(cfg), C.byref(self._s)))
+        self._L = lib()  # kept for close() at interpreter exit
+        check(self._L.lqg_serve_start(index._h, C.byref(cfg), C.byref(self._s)))
 
     def submit(self, query, ef: int, k: int) -> int | None:
         """Ticket, or None when every slot is taken (retry later)."""
@@ -82,8 +83,9 @@ class Server:
 
     def close(self) -> None:
         s = getattr(self, "_s", None)
-        if s is not None and s.value:
-            lib().lqg_serve_stop(s)
+        L = getattr(self, "_L", None)
+        if s is not None and s.value and L is not None:
+            L.lqg_serve_stop(s)
             self._s = None
 
     def __enter__(self):
@@ -110,8 +112,9 @@ class BatchGraph:
                  order: str = "nearest", d_ef: int | None = None, d_k: int | None = None,
                  d_dists: int | None = None, d_stats: int | None = None):
         self._g = C.c_void_p()
+        self._L = lib()  # kept for close() at interpreter exit
         vp = lambda x: None if x is None else C.c_void_p(x)  # noqa: E731
-        check(lib().lqg_graph_create(index._h, vp(d_queries), int(nq), vp(d_ef), vp(d_k),
+        check(self._L.lqg_graph_create(index._h, vp(d_queries), int(nq), vp(d_ef), vp(d_k),
                                      int(ef_max), int(kmax), ORDER[order], vp(d_results),
                                      vp(d_dists), vp(d_stats), C.byref(self._g)))
 
@@ -120,8 +123,9 @@ class BatchGraph:
 
     def close(self) -> None:
         g = getattr(self, "_g", None)
-        if g is not None and g.value:
-            lib().lqg_graph_destroy(g)
+        L = getattr(self, "_L", None)
+        if g is not None and g.value and L is not None:
+            L.lqg_graph_destroy(g)
             self._g = None
 
     def __del__(self):

@@ -47,8 +47,12 @@ class Index:
     def __init__(self, index_type: str, metric: str, num_elements: int, dimension: int,
                  degree_bound: int = 32):
         self._h = C.c_void_p()
-        st = lib().lqg_create(str(index_type).encode(), str(metric).encode(), int(num_elements),
-                              int(dimension), int(degree_bound), C.byref(self._h))
+        # the library object is kept here so the handle can be released even at
+        # interpreter exit, when module globals may already be cleared
+        self._L = lib()
+        st = self._L.lqg_create(str(index_type).encode(), str(metric).encode(),
+                                int(num_elements), int(dimension), int(degree_bound),
+                                C.byref(self._h))
         if st != 0:
             raise ValueError(lib().lqg_last_error().decode())
         self.num_elements = int(num_elements)
@@ -58,8 +62,9 @@ class Index:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            lib().lqg_destroy(h)
+        L = getattr(self, "_L", None)
+        if h is not None and h.value and L is not None:
+            L.lqg_destroy(h)
             self._h = None
 
     # -- reference API -----------------------------------------------------

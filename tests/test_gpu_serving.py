@@ -1,7 +1,8 @@
 """HIP-graph replays and the serving engine on the device.
 
 Bar: every graph replay equals the eager batch launch (replayed several
-times with eager work and synchronisations in between); every served
+times with eager work and synchronisations in between, and two graphs
+replayed in turn on two streams so their launches overlap); every served
 request's ids and distances equal those of the same query searched alone,
 whether it was served alone or in a mix of thousands of requests with other
 ef and k; the engine really batches (batches > 1 request) and overlaps
@@ -74,6 +75,40 @@ def test_graph_replays_equal_eager(tmp_path, fastscan):
     for i in range(nq):
         assert np.array_equal(got[i, :kk[i]], want2[i, :kk[i]])
     g.close()
+
+
+@pytest.mark.parametrize("fastscan", [False, True], ids=["sq8", "fastscan"])
+def test_graphs_back_to_back_on_two_streams(tmp_path, fastscan):
+    """Two graphs of the same batch, each with its own results and scratch,
+    replayed in turn on two streams with no synchronisation between them, so
+    their launches overlap on the device: every replay equals the eager
+    launch."""
+    import torch
+
+    from liteqg_amd.serving import BatchGraph
+
+    ix, arr = _index(tmp_path, fastscan=fastscan)
+    rng = np.random.default_rng(3)
+    nq, k, ef = 2000, 10, 64
+    Q = rng.random((nq, arr.dim), dtype=np.float32)
+    want = ix.search_batch(Q, k, ef=ef, order="nearest")
+    dq = torch.from_numpy(Q).cuda()
+    outs = [torch.full((nq, k), -1, dtype=torch.int32, device="cuda") for _ in range(2)]
+    graphs = [BatchGraph(ix, dq.data_ptr(), nq, o.data_ptr(), k, ef) for o in outs]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    try:
+        for rnd in range(3):
+            for o in outs:
+                o.fill_(-1)
+            torch.cuda.synchronize()
+            for i in range(6):
+                graphs[i % 2].launch(streams[i % 2].cuda_stream)
+            torch.cuda.synchronize()
+            for j, o in enumerate(outs):
+                assert np.array_equal(o.cpu().numpy().view(np.uint32), want), (rnd, j)
+    finally:
+        for g in graphs:
+            g.close()
 
 
 @pytest.mark.parametrize("fastscan", [False, True], ids=["sq8", "fastscan"])
