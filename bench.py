@@ -42,7 +42,7 @@ FS_EF_SWEEP = (50, 100, 128, 150, 200, 300, 500, 700, 1000)
 SERVE_RATES = (100_000.0, 1_000_000.0, 4_000_000.0)
 SERVE_LANES, SERVE_BATCH, SERVE_SLOTS = 4, 2048, 16384
 SERVE_MIX_RATES = (5_000.0, 20_000.0, 50_000.0)
-SERVE_MIX_LANES, SERVE_MIX_BATCH = 4, 2048
+SERVE_MIX_LANES, SERVE_MIX_BATCH = 16, 2048
 TARGET_RECALL = 0.95
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "h1_search_kernel_c2_traffic.md")
 TRAFFIC_FILE_FS = os.path.join(ROOT, "profiles", "h2_search_kernel_c2_traffic.md")
