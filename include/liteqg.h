@@ -159,7 +159,9 @@ lqg_status lqg_graph_destroy(lqg_batch_graph* graph);
 
 /* Serving engine: requests (one query each, its own ef and k) are submitted
  * as they arrive and complete asynchronously.  A dispatcher thread keeps up to
- * `lanes` batch launches in flight on separate streams; requests that arrive
+ * `lanes` batch launches in flight on separate streams, each stream with a
+ * hardware queue of its own (a CU-masked stream over every CU, so launches of
+ * different lanes do not wait for each other in a shared queue); requests that arrive
  * while launches run are admitted to the next free lane (continuous
  * batching), so their work overlaps the work already on the device.
  * Admission: at most `slots` requests outstanding (host pinned request
