@@ -477,6 +477,7 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
         job.out_stats[2 * static_cast<size_t>(q)] = n_exp;
         job.out_stats[2 * static_cast<size_t>(q) + 1] = n_eval;
     }
+    if (job.done) signal_host(job.done + q, lane);
     wave_sync();
 }
 

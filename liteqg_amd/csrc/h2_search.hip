@@ -728,6 +728,8 @@ __device__ void h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
         }
         if (vis.failed) atomicOr(job.err, 1u);
     }
+    // a query whose visited set overflowed gets no flag: its batch fails as a whole
+    if (job.done && !vis.failed) signal_host(job.done + q, lane);
     wave_sync();
 }
 

@@ -158,6 +158,7 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
     job.visit_bits = nullptr;
     job.visit_words = 0;
     job.visit_waves = 0;
+    job.done = sc->done;
     const int cls = ef_class(ef_max);
     LaunchCfg cfg = h->launch_set[class_slot(cls)] ? h->launch[class_slot(cls)] : default_launch(cls);
     cfg.ef_cap = cls;

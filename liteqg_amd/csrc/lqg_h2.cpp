@@ -201,6 +201,7 @@ lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const
     job.gvis = sc->gvis.p;
     job.gvis_bits = kGvisBits;
     job.err = sc->err.p;
+    job.done = sc->done;
     job.rot = h->h2_rotation == LQG_ROTATION_GEMM ? sc->rot.p : nullptr;
     LQG_HIP_TRY(launch_reset_counter(sc->counter.p, s));
     LQG_HIP_TRY(launch_reset_counter(sc->err.p, s));

@@ -50,6 +50,9 @@ struct Scratch {
     DevBuf<uint32_t> err;      // FastScan visited-set overflow flag
     DevBuf<uint32_t> gvis;     // FastScan per-wave overflow visited tables
     DevBuf<float> rot;         // FastScan queries rotated by the GEMM (nq x P)
+    // Per-query completion flags the launch sets (host-coherent memory owned by
+    // the caller, e.g. a serving lane), or null.
+    uint32_t* done = nullptr;
     void release() {
         counter.release();
         err.release();

@@ -49,6 +49,10 @@ struct SearchJob {
     // Queries the integer-path launch left to the float path (counted by the
     // integer launch, read by the float launch, which returns when it is 0).
     uint32_t* left = nullptr;
+    // Per-query completion flags in host-coherent memory, or null: the launch
+    // that writes a query's results then sets its flag to 1 (after a
+    // system-scope fence), so a host can take each result as soon as it exists.
+    uint32_t* done = nullptr;
 };
 
 // Launch geometry chosen by the host (autotuner or defaults).
@@ -90,6 +94,7 @@ struct H2Job {
     uint32_t gvis_bits;
     uint32_t* err;          // set to 1 if a visited set overflows both tables
     const float* rot = nullptr;  // nq x p queries rotated beforehand (GEMM), or null (FHT)
+    uint32_t* done = nullptr;    // per-query completion flags, as in SearchJob
 };
 
 struct H2Cfg {

@@ -2,16 +2,18 @@
 // batched over several batch launches in flight on separate streams.
 //
 // Requests live in a fixed table of slots (query in pinned host memory, ef,
-// k, results).  One dispatcher thread loops: it retires launches whose
-// completion event has fired (copying each request's results to its slot),
-// and whenever a lane (stream + device buffers + launch scratch) is idle and
-// requests are pending, it gathers up to max_batch of them into the lane's
-// pinned staging buffer and enqueues copy-in, the search launch and
-// copy-out on the lane's stream.  A request arriving while launches run is
-// therefore admitted to the next free lane without waiting for the others;
-// with several lanes busy, several batches occupy the device at once.  Each
-// query is walked by one wavefront independently of the others, so a
-// request's results do not depend on its batch.
+// k, results).  Whenever a lane (stream + device buffers + launch scratch) is
+// idle and requests are pending, the dispatcher thread gathers up to
+// max_batch of them into the lane's pinned staging buffer and enqueues
+// copy-in and the search launch on the lane's stream.  The kernels write each
+// request's results straight into the lane's host-coherent result buffers
+// and then set the request's flag, so the dispatcher hands every request
+// back as soon as its own walk ends, not when its batch's slowest request
+// does; the batch's completion event frees the lane.  A request arriving
+// while launches run is admitted to the next free lane without waiting for
+// the others; with several lanes busy, several batches occupy the device at
+// once.  Each query is walked by one wavefront independently of the others,
+// so a request's results do not depend on its batch.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -43,14 +45,21 @@ struct Lane {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     lqg::Scratch scratch;
-    lqg::DevBuf<float> d_q, d_dist;
-    lqg::DevBuf<uint32_t> d_ef, d_k, d_ids;
+    lqg::DevBuf<float> d_q;
+    lqg::DevBuf<uint32_t> d_ef, d_k;
     float* h_q = nullptr;      // pinned staging: max_batch x dim
     uint32_t* h_ek = nullptr;  // pinned: ef[max_batch] | k[max_batch]
-    uint32_t* h_ids = nullptr; // pinned: max_batch x kmax
-    float* h_dist = nullptr;   // pinned: max_batch x kmax
-    uint32_t* h_err = nullptr; // pinned: FastScan overflow flag
+    // host-coherent, written by the kernels: results and per-request flags
+    uint32_t* h_ids = nullptr;  // max_batch x kmax
+    float* h_dist = nullptr;    // max_batch x kmax
+    uint32_t* h_flag = nullptr; // max_batch
+    uint32_t* g_ids = nullptr;  // the same three as device addresses
+    float* g_dist = nullptr;
+    uint32_t* g_flag = nullptr;
+    uint32_t* h_err = nullptr;  // pinned: FastScan overflow flag
     std::vector<uint32_t> slots;
+    std::vector<uint8_t> out;   // request i of the batch already handed back
+    size_t n_out = 0;
     bool busy = false;
     bool fastscan = false;
     bool failed = false;
@@ -84,14 +93,13 @@ void free_lane(Lane& l) {
     if (l.stream) (void)hipStreamSynchronize(l.stream);
     l.scratch.release();
     l.d_q.release();
-    l.d_dist.release();
     l.d_ef.release();
     l.d_k.release();
-    l.d_ids.release();
     if (l.h_q) (void)hipHostFree(l.h_q);
     if (l.h_ek) (void)hipHostFree(l.h_ek);
     if (l.h_ids) (void)hipHostFree(l.h_ids);
     if (l.h_dist) (void)hipHostFree(l.h_dist);
+    if (l.h_flag) (void)hipHostFree(l.h_flag);
     if (l.h_err) (void)hipHostFree(l.h_err);
     if (l.done) (void)hipEventDestroy(l.done);
     if (l.stream) (void)hipStreamDestroy(l.stream);
@@ -136,6 +144,11 @@ void launch(lqg_server* s, Lane& l, const std::vector<uint32_t>& batch) {
         l.h_ek[mb + i] = s->k[slot];
     }
     l.slots = batch;
+    l.out.assign(b, 0);
+    l.n_out = 0;
+    // the lane's previous launch has completed, so nothing writes these now
+    memset(l.h_flag, 0, 4ull * b);
+    l.scratch.done = l.g_flag;
     l.failed = false;
     l.fastscan = s->h->mode == LQG_MODE_FASTSCAN;
     bool ok = hipMemcpyAsync(l.d_q.p, l.h_q, 4ull * b * d, hipMemcpyHostToDevice, l.stream) ==
@@ -145,11 +158,8 @@ void launch(lqg_server* s, Lane& l, const std::vector<uint32_t>& batch) {
               hipMemcpyAsync(l.d_k.p, l.h_ek + mb, 4ull * b, hipMemcpyHostToDevice, l.stream) ==
                   hipSuccess;
     ok = ok && lqg::run_batch(s->h, l.d_q.p, b, l.d_ef.p, l.d_k.p, s->cfg.ef_max, s->cfg.kmax,
-                              s->cfg.order, l.d_ids.p, l.d_dist.p, nullptr, l.stream, false,
+                              s->cfg.order, l.g_ids, l.g_dist, nullptr, l.stream, false,
                               &l.scratch) == LQG_OK;
-    const size_t rb = 4ull * b * s->cfg.kmax;
-    ok = ok && hipMemcpyAsync(l.h_ids, l.d_ids.p, rb, hipMemcpyDeviceToHost, l.stream) == hipSuccess &&
-         hipMemcpyAsync(l.h_dist, l.d_dist.p, rb, hipMemcpyDeviceToHost, l.stream) == hipSuccess;
     if (ok && l.fastscan)
         ok = hipMemcpyAsync(l.h_err, l.scratch.err.p, 4, hipMemcpyDeviceToHost, l.stream) ==
              hipSuccess;
@@ -158,25 +168,52 @@ void launch(lqg_server* s, Lane& l, const std::vector<uint32_t>& batch) {
     l.busy = true;
 }
 
-// Copies a finished lane's results to its requests' slots.
+// Request i of lane l's batch, whose results are in the lane's buffers, is
+// done (or failed): copy them to its slot.  Caller holds s->mu.
+void hand_back(lqg_server* s, Lane& l, size_t i, bool bad, int64_t t) {
+    const uint32_t slot = l.slots[i], km = s->cfg.kmax;
+    if (!bad) {
+        memcpy(&s->ids[static_cast<size_t>(slot) * km], l.h_ids + i * km, 4ull * km);
+        memcpy(&s->dist[static_cast<size_t>(slot) * km], l.h_dist + i * km, 4ull * km);
+    }
+    s->t_done[slot] = t;
+    s->state[slot].store(bad ? kFailed : kDone, std::memory_order_release);
+    l.out[i] = 1;
+    ++l.n_out;
+    ++s->st.completed;
+}
+
+// Hands back the requests of a running lane whose flags the kernels have set
+// (their results were written before the flag).  True if any.
+bool deliver(lqg_server* s, Lane& l, std::vector<uint32_t>& ready) {
+    if (l.failed || l.n_out == l.slots.size()) return false;
+    ready.clear();
+    for (size_t i = 0; i < l.slots.size(); ++i)
+        if (!l.out[i] && __atomic_load_n(&l.h_flag[i], __ATOMIC_ACQUIRE) != 0u)
+            ready.push_back(static_cast<uint32_t>(i));
+    if (ready.empty()) return false;
+    const int64_t t = now_ns();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        for (uint32_t i : ready) hand_back(s, l, i, false, t);
+    }
+    s->cv_done.notify_all();
+    return true;
+}
+
+// The lane's launch has completed: hands back what is left (all of it failed
+// if the launch did, or if a FastScan visited set overflowed) and frees it.
 void retire(lqg_server* s, Lane& l) {
     // a launch that failed to enqueue may still have copies in flight that
     // read the lane's staging buffers
     if (l.failed) (void)hipStreamSynchronize(l.stream);
     const bool bad = l.failed || (l.fastscan && l.h_err[0] != 0);
-    const uint32_t km = s->cfg.kmax;
     const int64_t t = now_ns();
-    std::lock_guard<std::mutex> g(s->mu);
-    for (size_t i = 0; i < l.slots.size(); ++i) {
-        const uint32_t slot = l.slots[i];
-        if (!bad) {
-            memcpy(&s->ids[static_cast<size_t>(slot) * km], l.h_ids + i * km, 4ull * km);
-            memcpy(&s->dist[static_cast<size_t>(slot) * km], l.h_dist + i * km, 4ull * km);
-        }
-        s->t_done[slot] = t;
-        s->state[slot].store(bad ? kFailed : kDone, std::memory_order_release);
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        for (size_t i = 0; i < l.slots.size(); ++i)
+            if (!l.out[i]) hand_back(s, l, i, bad, t);
     }
-    s->st.completed += l.slots.size();
     l.slots.clear();
     l.busy = false;
     s->cv_done.notify_all();
@@ -184,12 +221,13 @@ void retire(lqg_server* s, Lane& l) {
 
 void dispatcher(lqg_server* s) {
     (void)hipSetDevice(s->device);
-    std::vector<uint32_t> batch;
+    std::vector<uint32_t> batch, ready;
     for (;;) {
         bool progressed = false;
         uint32_t busy = 0;
         for (Lane& l : s->lanes) {
             if (!l.busy) continue;
+            if (deliver(s, l, ready)) progressed = true;
             if (l.failed || hipEventQuery(l.done) == hipSuccess) {
                 retire(s, l);
                 progressed = true;
@@ -329,14 +367,22 @@ lqg_status lqg_serve_start(lqg_index* h, const lqg_serve_config* cfg, lqg_server
         e = lane_stream(&l.stream);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&l.done, hipEventDisableTiming);
         if (e == hipSuccess) e = l.d_q.reserve(mb * d);
-        if (e == hipSuccess) e = l.d_dist.reserve(mb * km);
-        if (e == hipSuccess) e = l.d_ids.reserve(mb * km);
         if (e == hipSuccess) e = l.d_ef.reserve(mb);
         if (e == hipSuccess) e = l.d_k.reserve(mb);
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&l.h_q), 4 * mb * d);
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&l.h_ek), 8 * mb);
-        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&l.h_ids), 4 * mb * km);
-        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&l.h_dist), 4 * mb * km);
+        // written by the kernels and read by this thread while they run
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void**>(&l.h_ids), 4 * mb * km, hipHostMallocCoherent);
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void**>(&l.h_dist), 4 * mb * km, hipHostMallocCoherent);
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void**>(&l.h_flag), 4 * mb, hipHostMallocCoherent);
+        if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&l.g_ids), l.h_ids, 0);
+        if (e == hipSuccess)
+            e = hipHostGetDevicePointer(reinterpret_cast<void**>(&l.g_dist), l.h_dist, 0);
+        if (e == hipSuccess)
+            e = hipHostGetDevicePointer(reinterpret_cast<void**>(&l.g_flag), l.h_flag, 0);
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&l.h_err), 4);
         if (e == hipSuccess) l.h_err[0] = 0;
     }

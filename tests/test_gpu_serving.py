@@ -163,3 +163,28 @@ def test_server_admission_and_errors(tmp_path):
         assert np.array_equal(srv.wait(extra, 10)[0], got[0])
         with pytest.raises(LqgError):
             srv.poll(tickets[0], 10)  # already released
+
+
+@pytest.mark.parametrize("fastscan", [False, True], ids=["sq8", "fastscan"])
+def test_requests_complete_before_their_batch(tmp_path, fastscan):
+    """Each request is handed back when its own walk ends (the kernels write
+    its results and then its flag), not when its batch's slowest request does:
+    with one request at ef 1024 among requests at ef 8, the fast ones finish
+    well before it, and every result equals the same query searched alone."""
+    from liteqg_amd.serving import Server
+
+    ix, arr = _index(tmp_path, fastscan=fastscan)
+    rng = np.random.default_rng(9)
+    nq = 400
+    Q = rng.random((nq, arr.dim), dtype=np.float32)
+    ef = np.full(nq, 8, np.uint32)
+    ef[0] = 1024
+    k = np.full(nq, 10, np.uint32)
+    want = ix.search_batch(Q, k, ef=ef, order="nearest")
+    with Server(ix, lanes=2, max_batch=nq, slots=nq, ef_max=1024, kmax=10) as srv:
+        srv.replay(Q[:8], ef[:8], k[:8], np.zeros(8, np.uint64))  # warm-up
+        lat, ids = srv.replay(Q, ef, k, np.zeros(nq, np.uint64))
+        st = srv.stats()
+    assert np.array_equal(ids, want)
+    assert st["completed"] == st["submitted"]
+    assert np.median(lat[1:]) < 0.5 * lat[0], (np.median(lat[1:]), lat[0])
