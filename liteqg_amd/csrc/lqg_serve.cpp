@@ -318,6 +318,32 @@ lqg_status submit_impl(lqg_server* s, const float* query, uint32_t ef, uint32_t 
     return LQG_OK;
 }
 
+// Submits requests [0, n) of a replay under one lock, as many as there are
+// free slots; returns how many were taken (their tickets are written).
+uint64_t submit_due(lqg_server* s, const float* queries, const uint32_t* ef, const uint32_t* k,
+                    uint64_t n, uint64_t* tickets) {
+    uint64_t m = 0;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->stop) return 0;
+        for (; m < n && !s->free_list.empty(); ++m) {
+            const uint32_t slot = s->free_list.back();
+            s->free_list.pop_back();
+            ++s->gen[slot];
+            memcpy(s->q_store + static_cast<size_t>(slot) * s->dim, queries + m * s->dim,
+                   4ull * s->dim);
+            s->ef[slot] = ef[m];
+            s->k[slot] = k[m];
+            s->state[slot].store(kPending, std::memory_order_relaxed);
+            s->pending.push_back(slot);
+            ++s->st.submitted;
+            tickets[m] = (static_cast<uint64_t>(s->gen[slot]) << 32) | slot;
+        }
+    }
+    if (m) s->cv_work.notify_one();
+    return m;
+}
+
 }  // namespace
 
 extern "C" {
@@ -457,7 +483,8 @@ lqg_status lqg_serve_replay(lqg_server* s, const float* queries, uint64_t nq, co
     for (uint64_t i = 0; i < nq; ++i) ready[i].store(0);
     std::atomic<int> failed{0};
     const int64_t t0 = now_ns();
-    // collector: waits for the tickets in submission order
+    // collector: waits for the tickets in submission order, watching the
+    // slot's state without the lock until the request is done
     std::thread collector([&] {
         std::vector<uint32_t> buf(km);
         for (uint64_t i = 0; i < nq; ++i) {
@@ -466,6 +493,12 @@ lqg_status lqg_serve_replay(lqg_server* s, const float* queries, uint64_t nq, co
                 failed.store(1);
                 latency_ns[i] = 0;
                 continue;
+            }
+            const uint32_t slot = static_cast<uint32_t>(tickets[i] & 0xFFFFFFFFu);
+            for (;;) {
+                const int st = s->state[slot].load(std::memory_order_acquire);
+                if (st == kDone || st == kFailed) break;
+                std::this_thread::yield();
             }
             int64_t td = 0;
             uint32_t* out = results ? results + i * km : buf.data();
@@ -476,22 +509,29 @@ lqg_status lqg_serve_replay(lqg_server* s, const float* queries, uint64_t nq, co
             latency_ns[i] = static_cast<uint64_t>(td > sched ? td - sched : 0);
         }
     });
-    for (uint64_t i = 0; i < nq; ++i) {
-        const int64_t at = t0 + static_cast<int64_t>(arrival_ns[i]);
-        while (now_ns() < at) {
-        }
-        for (;;) {
-            const lqg_status st = submit_impl(s, queries + i * s->dim, ef[i], k[i], &tickets[i]);
-            if (st == LQG_OK) {
-                ready[i].store(1, std::memory_order_release);
-                break;
+    // submitter: every request due by now goes in under one lock (requests
+    // behind schedule are submitted together), until the slots run out
+    for (uint64_t i = 0; i < nq;) {
+        const int64_t now = now_ns();
+        uint64_t j = i;
+        while (j < nq && t0 + static_cast<int64_t>(arrival_ns[j]) <= now) ++j;
+        if (j == i) continue;  // the next request is not due yet
+        const uint64_t m =
+            submit_due(s, queries + i * s->dim, ef + i, k + i, j - i, tickets.data() + i);
+        for (uint64_t t = i; t < i + m; ++t) ready[t].store(1, std::memory_order_release);
+        if (m == 0) {
+            bool stopping;
+            {
+                std::lock_guard<std::mutex> g(s->mu);
+                stopping = s->stop;
             }
-            if (st != LQG_ERR_BUSY) {
-                ready[i].store(-1, std::memory_order_release);
+            if (stopping) {
+                for (uint64_t t = i; t < nq; ++t) ready[t].store(-1, std::memory_order_release);
                 break;
             }
             std::this_thread::yield();
         }
+        i += m;
     }
     collector.join();
     if (failed.load()) return fail(LQG_ERR_DEVICE, "some replayed requests failed");
