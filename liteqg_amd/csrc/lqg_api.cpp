@@ -128,7 +128,7 @@ LaunchCfg default_launch(int ef_cap) {
 lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
                      const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, lqg_order order,
                      uint32_t* d_results, float* d_dists, uint32_t* d_stats, hipStream_t s,
-                     bool count_visits, Scratch* sc) {
+                     bool count_visits, Scratch* sc, int queries_int) {
     if (!sc) sc = &h->scratch;
     if (h->mode == LQG_MODE_FASTSCAN) {
         if (count_visits) return fail(LQG_ERR_UNSUPPORTED, "visit counting is for the SQ8 path");
@@ -186,6 +186,25 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
     }
     LQG_HIP_TRY(launch_reset_counter(sc->counter.p, s, static_cast<uint32_t>(nwords)));
     uint32_t* left = sc->counter.p + 2 * kQueueWords;
+    // queries_int: the caller has checked on the host that every query is
+    // (1) or that none is (0) integer valued in [0, 255]; then only the
+    // launch that takes them runs.  -1: unknown, both run.
+    if (int_index && queries_int == 1 && !count_visits) {
+        cfg.eval = int_eval;
+        job.counter = sc->counter.p;
+        job.skip_int = 0;
+        job.left = nullptr;
+        LQG_HIP_TRY(launch_search(h->dev(), job, cfg, s));
+        return LQG_OK;
+    }
+    if (int_index && queries_int == 0 && !count_visits) {
+        cfg.eval = 0;
+        job.skip_int = 0;
+        job.left = nullptr;
+        job.counter = sc->counter.p + kQueueWords;
+        LQG_HIP_TRY(launch_search(h->dev(), job, cfg, s));
+        return LQG_OK;
+    }
     if (int_index) {
         cfg.eval = int_eval;
         job.counter = sc->counter.p;
@@ -250,6 +269,9 @@ lqg_status lqg_create(const char* index_type, const char* metric, uint64_t num_e
 
 lqg_status lqg_destroy(lqg_index* h) {
     if (!h) return LQG_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->p_q) (void)hipHostFree(h->p_q);
+    if (h->p_out) (void)hipHostFree(h->p_out);
     h->d_qdata.release();
     h->d_nbr.release();
     h->d_min.release();
@@ -382,15 +404,64 @@ lqg_status lqg_search(lqg_index* h, const float* query, uint32_t k, uint32_t* re
     if (st != LQG_OK) return st;
     st = lqg::ensure_device(h);
     if (st != LQG_OK) return st;
-    HIP_TRY(h->d_q.reserve(h->dim));
-    HIP_TRY(h->d_ids.reserve(k));
-    HIP_TRY(hipMemcpyAsync(h->d_q.p, query, h->dim * 4, hipMemcpyHostToDevice, h->stream));
-    st = lqg::run_batch(h, h->d_q.p, 1, nullptr, nullptr, h->ef, k, h->order, h->d_ids.p, nullptr,
-                   nullptr, h->stream);
-    if (st != LQG_OK) return st;
-    HIP_TRY(hipMemcpyAsync(results, h->d_ids.p, k * 4ull, hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    return lqg::h2_check_error(h);
+    // The kernel reads the query from pinned host memory and writes the ids and
+    // a completion flag back there: no copies through pageable memory.  The
+    // stream is still synchronised before returning (what is left of it then
+    // is at most the float-path launch finding nothing to do), so no later
+    // call can free memory a launch of this one still reads.
+    if (!h->p_q) {
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&h->p_q), 4 * h->dim, hipHostMallocCoherent));
+        HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&h->g_pq), h->p_q, 0));
+    }
+    if (h->p_out_k < k) {
+        if (h->p_out) {
+            HIP_TRY(hipStreamSynchronize(h->stream));
+            HIP_TRY(hipHostFree(h->p_out));
+            h->p_out = nullptr;
+            h->p_out_k = 0;
+        }
+        HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&h->p_out), 4ull * (16 + k),
+                              hipHostMallocCoherent));
+        HIP_TRY(hipHostGetDevicePointer(reinterpret_cast<void**>(&h->g_pout), h->p_out, 0));
+        h->p_out_k = k;
+    }
+    memcpy(h->p_q, query, 4 * h->dim);
+    __atomic_store_n(&h->p_out[0], 0u, __ATOMIC_RELEASE);
+    // the integer-path test of the kernels, on the host: one launch instead of two
+    int qint = 1;
+    for (uint64_t i = 0; i < h->dim; ++i) {
+        const float v = query[i];
+        if (!(v >= 0.f && v <= 255.f && v == std::rint(v))) {
+            qint = 0;
+            break;
+        }
+    }
+    h->scratch.done = h->g_pout;
+    st = lqg::run_batch(h, h->g_pq, 1, nullptr, nullptr, h->ef, k, h->order, h->g_pout + 16,
+                        nullptr, nullptr, h->stream, false, nullptr, qint);
+    h->scratch.done = nullptr;
+    if (st != LQG_OK) {
+        (void)hipStreamSynchronize(h->stream);
+        return st;
+    }
+    // wait for the flag; a stream that finishes without it had a query fail
+    // (a FastScan visited set that overflowed) or a launch error
+    for (uint32_t spin = 1;; ++spin) {
+        if (__atomic_load_n(&h->p_out[0], __ATOMIC_ACQUIRE) != 0u) {
+            memcpy(results, h->p_out + 16, 4ull * k);
+            HIP_TRY(hipStreamSynchronize(h->stream));
+            return LQG_OK;
+        }
+        if ((spin & 255u) == 0u) {
+            const hipError_t q = hipStreamQuery(h->stream);
+            if (q == hipErrorNotReady) continue;
+            if (__atomic_load_n(&h->p_out[0], __ATOMIC_ACQUIRE) != 0u) continue;
+            HIP_TRY(q);
+            HIP_TRY(hipStreamSynchronize(h->stream));
+            st = lqg::h2_check_error(h);
+            return st != LQG_OK ? st : fail(LQG_ERR_DEVICE, "search finished without a result");
+        }
+    }
 }
 
 lqg_status lqg_search_batch(lqg_index* h, const float* queries, uint64_t nq, const uint32_t* ef,

@@ -104,6 +104,14 @@ struct lqg_index {
     lqg::Scratch scratch;  // for the entry points that run on the handle's stream
     lqg::DevBuf<float> d_dist;
     lqg::DevBuf<uint32_t> d_visit;  // exact visit counting bitmaps
+    // plain search (one query per call): pinned, host-coherent buffers the
+    // kernels read the query from and write the ids and the completion flag
+    // to (word 0 of p_out, ids from word 16), and their device addresses
+    float* p_q = nullptr;
+    uint32_t* p_out = nullptr;
+    uint32_t p_out_k = 0;  // ids p_out holds
+    float* g_pq = nullptr;
+    uint32_t* g_pout = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // launch settings per pool-capacity class (64, 128, 192, 256, 512, 1024)
     lqg::LaunchCfg launch[lqg::kClasses];
@@ -162,7 +170,7 @@ lqg_status ensure_device(lqg_index* h);
 lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
                      const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, lqg_order order,
                      uint32_t* d_results, float* d_dists, uint32_t* d_stats, hipStream_t s,
-                     bool count_visits = false, Scratch* sc = nullptr);
+                     bool count_visits = false, Scratch* sc = nullptr, int queries_int = -1);
 
 // FastScan counterpart of run_batch (lqg_h2.cpp); d_stats gets (E, pops).
 lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,

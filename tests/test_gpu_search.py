@@ -289,3 +289,26 @@ def test_count_visits_with_tiny_cache(oracle_mod, tmp_path):
     assert np.array_equal(got, est.astype(np.int64))
     _, st = ix.search_batch(Q, 10, ef=100, order="nearest", return_stats=True)
     assert st["evaluations"] > int(got[:, 1].sum())
+
+
+def test_plain_search_on_integer_index(oracle_mod, tmp_path):
+    """One query per call on an index with exact SQ8 decode: an integer-valued
+    query runs the integer launch alone and any other the float launch alone
+    (the host makes the kernels' test); both equal the batched search and the
+    oracle, in both orders."""
+    arr, base = make_random_index(4000, 128, 32, seed=23, kind="int", knn=True)
+    ix = device_from_arrays(arr, tmp_path)
+    rng = np.random.default_rng(8)
+    Q = np.clip(np.rint(base[rng.integers(0, 4000, 24)] + rng.normal(0, 15, (24, 128))), 0, 255)
+    Q = Q.astype(np.float32)
+    Q[1::3, 5] += 0.5      # fractional
+    Q[2::6, 0] = 256.0     # out of byte range
+    for order in ("nearest", "reference"):
+        ix.set_order(order)
+        for ef in (10, 64):
+            ix.set_ef(ef)
+            got = np.stack([ix.search(q, 10) for q in Q])
+            want = ix.search_batch(Q, 10, ef=ef, order=order)
+            exp, _, _ = oracle_batch(oracle_mod, arr, Q, ef, 10, order)
+            assert np.array_equal(got, want), (order, ef)
+            assert np.array_equal(got, exp), (order, ef)
