@@ -318,3 +318,25 @@ def test_mode_switch_keeps_sq8_results(tmp_path):
     ix.search_batch(Q, 10, ef=50)
     ix.set_mode("sq8")
     assert np.array_equal(ix.search_batch(Q, 10, ef=50, order="nearest"), before)
+
+
+@pytest.mark.parametrize("rotation", ["fht", "gemm"])
+def test_plain_search_equals_batch(oracle_mod, tmp_path, rotation):
+    """One query per call in FastScan mode (the kernels read the query from
+    pinned host memory and write the ids back there) equals the batched
+    search with either rotation, and with the FHT also the oracle."""
+    arr, _ = make_random_index(1500, 96, 32, seed=31, knn=True)
+    ix = _index_from_arrays(arr, tmp_path, "p")
+    ix.build_sidecar(signs=_signs(ix.padded_dim, 96))
+    ix.set_mode("fastscan")
+    ix.set_rotation(rotation)
+    rng = np.random.default_rng(12)
+    Q = (arr.raw[rng.integers(0, 1500, 20)] + rng.normal(0, 0.05, (20, 96))).astype(np.float32)
+    for ef in (16, 100):
+        ix.set_ef(ef)
+        got = np.stack([ix.search(q, 10) for q in Q])
+        want = ix.search_batch(Q, 10, ef=ef)
+        assert np.array_equal(got, want), (rotation, ef)
+        if rotation == "fht":
+            exp, _, _ = _oracle_h2(oracle_mod, arr, ix, Q, ef, 10)
+            assert np.array_equal(got, exp), ef
