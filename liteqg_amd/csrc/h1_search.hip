@@ -582,6 +582,16 @@ __global__ void __launch_bounds__(256) h1_search_kernel(DevIndex ix, SearchJob j
     search_kernel_body<EF_CAP, EVAL>(ix, job, cache_bits, dim_pad, wave_bytes);
 }
 
+// The same with at least 4 waves per SIMD (at most 128 VGPRs): the default
+// for pools of up to 256 entries.  Left alone, the float-path kernel of the
+// 256 class takes 131 VGPRs and drops to 3 waves per SIMD.
+template <int EF_CAP, int EVAL>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+h1_search_kernel_w4(DevIndex ix, SearchJob job, uint32_t cache_bits, uint32_t dim_pad,
+                    uint32_t wave_bytes) {
+    search_kernel_body<EF_CAP, EVAL>(ix, job, cache_bits, dim_pad, wave_bytes);
+}
+
 // Register file capped at 64 VGPRs so 8 waves fit per SIMD (32 per CU): more
 // queries in flight to hide memory latency, at the price of spills where the
 // code needs more.
@@ -610,7 +620,10 @@ template <int EF_CAP, int EVAL>
 KernelFn kernel_for(int occ) {
     if (occ == 8) return &h1_search_kernel_w8<EF_CAP, EVAL>;
     if (occ == 6) return &h1_search_kernel_w6<EF_CAP, EVAL>;
-    return &h1_search_kernel<EF_CAP, EVAL>;
+    if constexpr (EF_CAP <= 256)
+        return &h1_search_kernel_w4<EF_CAP, EVAL>;
+    else
+        return &h1_search_kernel<EF_CAP, EVAL>;
 }
 
 template <int EVAL>
