@@ -607,8 +607,7 @@ lqg_status lqg_get_launch(const lqg_index* h, uint32_t ef, lqg_launch* cfg) {
     if (cls == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024");
     const int slot = lqg::class_slot(cls);
     if (h->mode == LQG_MODE_FASTSCAN) {
-        const lqg::H2Cfg c =
-            h->h2_launch_set[slot] ? h->h2_launch[slot] : lqg::h2_default_launch(cls);
+        const lqg::H2Cfg c = lqg::h2_effective_launch(h, cls);
         cfg->waves_per_group = c.waves_per_group;
         cfg->cache_bits = c.hash_bits;
         cfg->occupancy = c.occupancy;

@@ -151,7 +151,22 @@ H2Cfg h2_default_launch(int ef_cap) {
     // LDS visited table: at least 4x the largest ef of the class, so that
     // expansions (typically 1x-3x ef) stay under half of it
     c.hash_bits = ef_cap <= 64 ? 9 : ef_cap <= 256 ? 10 : ef_cap <= 512 ? 11 : 12;
+    // 6 waves per SIMD up to the 192 class: with these defaults on the 1M x 128
+    // workload, 4-12% faster than uncapped at ef 50-150 (at ef 200 a 5-wave cap
+    // was 3% slower, so larger classes stay uncapped); a launch drops the cap
+    // where the index has no capped kernel (more than 4 code words per lane)
+    c.occupancy = ef_cap <= 192 ? 6 : 0;
     return c;
+}
+
+H2Cfg h2_effective_launch(const lqg_index* h, int ef_cap) {
+    const int slot = class_slot(ef_cap);
+    H2Cfg cfg = h->h2_launch_set[slot] ? h->h2_launch[slot] : h2_default_launch(ef_cap);
+    cfg.ef_cap = ef_cap;
+    if (!h->h2_launch_set[slot] && cfg.occupancy != 0 &&
+        !(h->h2_ready && h2_occupancy_ok(h->h2dev())))
+        cfg.occupancy = 0;
+    return cfg;
 }
 
 void h2_reset(lqg_index* h) {
@@ -166,9 +181,7 @@ lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const
     if (!h->h2_ready) return fail(LQG_ERR_STATE, "FastScan mode needs a sidecar");
     const int cls = ef_class(ef_max);
     if (cls == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported on the device");
-    const int slot = class_slot(cls);
-    H2Cfg cfg = h->h2_launch_set[slot] ? h->h2_launch[slot] : h2_default_launch(cls);
-    cfg.ef_cap = cls;
+    H2Cfg cfg = h2_effective_launch(h, cls);
     const H2Dev ix = h->h2dev();
     if (nq < static_cast<uint64_t>(cfg.waves_per_group)) cfg.waves_per_group = 1;
     while (cfg.waves_per_group > 1 && h2_lds_bytes(ix, cfg, kmax) > 160 * 1024)

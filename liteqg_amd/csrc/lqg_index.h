@@ -180,6 +180,10 @@ lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const
 // Launch settings of the FastScan kernel used when none were set for a class.
 H2Cfg h2_default_launch(int ef_cap);
 
+// The settings a FastScan launch of class ef_cap uses on this index: the ones
+// set, or the defaults without a register cap the index has no kernel for.
+H2Cfg h2_effective_launch(const lqg_index* h, int ef_cap);
+
 // Frees the FastScan records (index reloaded or rebuilt).
 void h2_reset(lqg_index* h);
 
