@@ -18,7 +18,23 @@ replayed back to back from four HIP graphs on four streams, so each batch's
 tail overlaps the next batch's start; ids are checked equal to the single
 launch's.  It shows the kernel's rate without the tail of a lone batch.
 
+``parity`` (the bench exits with status 3 after printing its line if any
+check fails): at full size, on a 2,000-query sample, the device's ids and
+distance bits against the oracle at every swept ef and both output orders,
+and the reference's own search() against the oracle's replica of its heap
+order; the FastScan leg checks its ids and distance bits against the oracle
+and, with the reference's own sidecar installed, its ids against the
+reference's search_qg on all 10,000 queries.
+
+``cpu_baseline``: the reference's search() (and, in the FastScan leg, its
+search_qg) compiled from its own headers, on every host core this process
+may use (affinity capped by the cgroup quota; count and CPU model recorded).
+
+``other_workloads``: the 100k x 64 and 10M x 96 configurations at full size
+(recall, QPS and roofline per ef); skipped with --no-configs.
+
 Usage:  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                        [--no-parity] [--no-configs] [--no-serve] [--no-fastscan]
 """
 from __future__ import annotations
 
@@ -50,6 +66,29 @@ TRAFFIC_FILE_FS = os.path.join(ROOT, "profiles", "h2_search_kernel_c2_traffic.md
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def host_cpus():
+    """Host cores this process may use: its CPU affinity, capped by the
+    cgroup's CPU quota where one is set; and the CPU model."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    threads = min(aff, quota) if quota else aff
+    return {"threads": threads, "affinity": aff, "cgroup_quota": quota, "model": model}
 
 
 def ground_truth(base: np.ndarray, Q: np.ndarray, k: int):
@@ -272,12 +311,14 @@ def serving_mix_leg(args):
     return out
 
 
-def configs_leg(args):
+def configs_leg(args, host):
     """Two more workloads at full size, SQ8 path, order=nearest, graphs built on
-    the device (3 iterations), FP64 ground truth, 10 timed launches per ef:
+    the device (3 iterations), FP64 ground truth, 10 timed launches per ef,
+    with the roofline of each launch (algorithmic bytes from the exact visit
+    counts, as for the headline workload):
       100,000 x 64 i.i.d. U[0,1) (the reference README's example), degree 32,
-        build EF 200, 1,000 queries, ef {50, 100, 200}, k 10; the plain path
-        and the reference's CPU search at ef 100 beside it;
+        build EF 200, 1,000 queries, ef {50, 100, 200}, k 10; the plain path,
+        the reference's CPU search at ef 100 and oracle parity beside it;
       10,000,000 x 96 unit-norm, 4,096 centres N(0, I), sigma 0.3 before
         normalisation, degree 32, build EF 300, 10,000 queries, ef {50, 200},
         k 10; the batch also replayed as a HIP graph."""
@@ -304,8 +345,14 @@ def configs_leg(args):
             torch.cuda.synchronize()
             rec = recall_at_k(db.results(), gt, 10)
             ms, _ = time_launches(db, ef, 10, 2)
+            ev = ix.count_visits(Q, ef, 10).astype(np.int64)
+            alg = float(np.sum(4 * d + 8 * d + 4 * 32 * ev[:, 0] + d * ev[:, 1] + 4 * 10))
+            gbs = alg / (ms / 1e3) / 1e9
             row = {"ef": ef, "recall": round(rec, 4), "qps": round(nq / ms * 1e3, 1),
-                   "ms_per_step": round(ms, 4)}
+                   "ms_per_step": round(ms, 4),
+                   "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                                "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                "bytes_per_query": round(alg / nq, 1)}}
             if name.startswith("unitmix"):
                 row["graph_replay_ms"] = round(graph_leg(ix, db, ef, args), 4)
             rows.append(row)
@@ -317,12 +364,25 @@ def configs_leg(args):
             for q in Q[:300]:
                 ix.search(q, 10)
             res["plain_qps_ef100"] = round(300 / (time.perf_counter() - t), 1)
-            if not args.no_cpu:
-                with tempfile.TemporaryDirectory() as tmp:
-                    p = os.path.join(tmp, "c.index")
-                    ix.save(p)
-                    res["cpu_reference_ef100"] = cpu_baseline(p, n, d, 32, Q, 100, 10,
-                                                              min(16, os.cpu_count() or 1))
+            with tempfile.TemporaryDirectory() as tmp:
+                p = os.path.join(tmp, "c.index")
+                ix.save(p)
+                if not args.no_parity:
+                    from oracle import oracle as O
+
+                    arr = O.read_index(p, n, d, 32)
+                    par = []
+                    for ef in efs:
+                        ids, dis = ix.search_batch(Q, 10, ef=ef, order="nearest",
+                                                   return_distances=True)
+                        e_ids, e_dis, _ = O.search_canon_batch(arr, Q, ef, 10, "nearest",
+                                                               threads=host["threads"])
+                        par.append(dict(ef=ef, ids_equal=bool(np.array_equal(ids, e_ids)),
+                                        dist_bits_equal=bool(np.array_equal(
+                                            dis.view(np.uint32), e_dis.view(np.uint32)))))
+                    res["parity_vs_oracle"] = par
+                if not args.no_cpu:
+                    res["cpu_reference_ef100"] = cpu_baseline(p, n, d, 32, Q, 100, 10, host)
         out[name] = res
         log(f"config {name}: {res}")
         del ix, db, base
@@ -330,12 +390,15 @@ def configs_leg(args):
     return out
 
 
-def fastscan_leg(ix, db, Q, gt, k, args):
+def fastscan_leg(ix, db, Q, gt, k, args, ctx):
     """The FastScan RaBitQ path (search_qg) on the same index and queries:
     device sidecar build, ef sweep to recall@10 >= 0.95, batched timing, and
     its roofline on algorithmic bytes per query
       4D (query) + E * (4D + deg*P/8 + 16*deg) (node records) + 4k (ids out)
-    with E the exact expansion count the kernel reports."""
+    with E the exact expansion count the kernel reports.  Then parity at full
+    size (the oracle's search_h2 on a sample; the reference's own search_qg on
+    its own sidecar for all queries) and the reference's search_qg timed on
+    the host cores as this path's CPU baseline."""
     import torch
 
     d, deg = ix.dimension, ix.degree_bound
@@ -396,13 +459,42 @@ def fastscan_leg(ix, db, Q, gt, k, args):
             "ids_overlap_with_fht": round(float(np.mean(
                 [len(set(a) & set(b)) / len(a) for a, b in zip(ids_fht, ids_gemm)])), 5)}
     log(f"fastscan gemm rotation: {gemm}")
+    # parity: the oracle on the device's sidecar, at the operating point and
+    # the ef below it in the sweep
+    par_efs = sorted({chosen, sweep[max(0, len(sweep) - 2)]["ef"]})
+    par = {"queries": int(ctx["sample"].shape[0]),
+           "vs_oracle": parity_fastscan(ix, ctx["arr"], ctx["sample"], par_efs, k, ctx["host"])}
+    log(f"fastscan parity vs oracle: {par['vs_oracle']}")
+    cpu = None
+    if not args.no_cpu:
+        cpu, r_ids, side = cpu_baseline_fastscan(ctx["path"], ix.num_elements, d, deg, signs, Q,
+                                                 chosen, k, ctx["host"], ctx["tmp"], True)
+        log(f"fastscan cpu baseline {cpu}")
+        if side is not None:
+            # the reference's own codes and factors on the device: its
+            # search_qg and this kernel then read the same bytes
+            ix.set_sidecar(signs, side[0], side[1], layout="blocks")
+            del side
+            ref_dev = ix.search_batch(Q, k, ef=chosen)
+            par["vs_reference_search_qg"] = {
+                "queries": int(Q.shape[0]), "ef": chosen,
+                "sidecar": "the reference's rabitq_codes rows, installed on the device",
+                "rows_identical": round(float(np.mean(np.all(ref_dev == r_ids, 1))), 5),
+                "rows_equal_up_to_ties": round(float(np.mean(
+                    same_up_to_ties(ctx["base"], Q, ref_dev, r_ids))), 5)}
+            log(f"fastscan parity vs reference: {par['vs_reference_search_qg']}")
+            ix.build_sidecar(seed=args.seed)  # back to the device-built sidecar
+    par["ok"] = all(r["ids_equal"] and r["dist_bits_equal"] for r in par["vs_oracle"]) and \
+        par.get("vs_reference_search_qg", {}).get("rows_equal_up_to_ties", 1.0) == 1.0
     ix.set_mode("sq8")
     log(f"fastscan timed ef={chosen}: {ms_step:.3f} ms/step, {qps:.0f} QPS, "
         f"{achieved:.0f} GB/s")
     return {"path": "FastScan RaBitQ search_qg (1-bit codes, exact re-rank)", "ef": chosen,
             "recall_at_10": rec, "qps": round(qps, 1), "ms_per_step": round(ms_step, 4),
             "ef_sweep": sweep, "sidecar_build_s": round(side_s, 2), "launch": tuned,
-            "gemm_rotation": gemm, "steady_state": steady,
+            "gemm_rotation": gemm, "steady_state": steady, "parity": par,
+            "cpu_baseline": cpu,
+            "gpu_over_cpu": round(qps / cpu["value"], 1) if cpu else None,
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "traffic": read_traffic(TRAFFIC_FILE_FS),
@@ -410,6 +502,100 @@ def fastscan_leg(ix, db, Q, gt, k, args):
                          "record_bytes": rec_bytes,
                          "expansions_per_query": round(float(E.mean()), 2),
                          "pops_per_query": round(float(pops.mean()), 2)}}
+
+
+def exact_dists(base, Q, ids):
+    """FP64 squared L2 from each query to each returned id (inf for
+    0xFFFFFFFF slots)."""
+    ok = ids != 0xFFFFFFFF
+    B = base[np.where(ok, ids, 0).astype(np.int64)].astype(np.float64)
+    d = ((B - Q[:, None, :].astype(np.float64)) ** 2).sum(-1)
+    return np.where(ok, d, np.inf)
+
+
+def same_up_to_ties(base, Q, a, b):
+    """Per query: the two id lists hold the same multiset of exact distances
+    (they may differ only where distances are equal)."""
+    da, db = np.sort(exact_dists(base, Q, a), 1), np.sort(exact_dists(base, Q, b), 1)
+    return np.all(da == db, axis=1)
+
+
+def ref_search_ids(ix_path, n, d, deg, Q, ef, k, tmp):
+    """The reference's own Index.search (set_ef, then search per query:
+    qg.hpp:226-243, k farthest of the final pool) via oracle/_ref/ref_driver."""
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.path.exists(drv):
+        return None
+    qp, op = os.path.join(tmp, "qpar.f32"), os.path.join(tmp, "ref_ids.u32")
+    np.ascontiguousarray(Q, np.float32).tofile(qp)
+    r = subprocess.run([drv, "search", ix_path, str(n), str(d), str(deg), qp, str(Q.shape[0]),
+                        str(ef), str(k), op], capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        log("ref_driver search failed:", r.returncode, r.stderr[-300:])
+        return None
+    return np.fromfile(op, np.uint32).reshape(Q.shape[0], k)
+
+
+def parity_sq8(ix, arr, base, sample, efs, k, host, ix_path, tmp):
+    """Full-size parity of the SQ8 path on a query sample:
+    - against the oracle's canonical search (orc_search_canon, keys ordered by
+      (distance, id)) at every swept ef, both output orders: ids and distance
+      bits must be identical;
+    - against the reference's own search() at every swept ef.  The device
+      orders equal distances by id, the reference by its heap's sift order
+      (libstdc++ push_heap / pop_heap), and where two distances tie the walk
+      itself can take another path.  The reference's ids must equal those of
+      the oracle's replica of its heap order (orc_search_heap, the canonical
+      search with the heap's tie order) on every query; the device's equal
+      the canonical search's (above).  The two oracles differ in nothing but
+      the order of equal keys, so every device/reference difference is a tie
+      difference.  Reported beside it: rows identical, and rows that hold the
+      same multiset of exact distances (exact here: the data are integers
+      0..255, so SQ8 is lossless and every FP32 sum is exact)."""
+    from oracle import oracle as O
+
+    oracle_rows, ref_rows = [], []
+    for ef in efs:
+        for order in ("reference", "nearest"):
+            ids, dis = ix.search_batch(sample, k, ef=ef, order=order, return_distances=True)
+            e_ids, e_dis, _ = O.search_canon_batch(arr, sample, ef, k, order,
+                                                   threads=host["threads"])
+            oracle_rows.append(dict(
+                ef=ef, order=order, ids_equal=bool(np.array_equal(ids, e_ids)),
+                dist_bits_equal=bool(np.array_equal(dis.view(np.uint32),
+                                                    e_dis.view(np.uint32)))))
+        r_ids = ref_search_ids(ix_path, arr.n, arr.dim, arr.deg, sample, ef, k, tmp)
+        if r_ids is not None:
+            ids = ix.search_batch(sample, k, ef=ef, order="reference")
+            h_ids = O.search_heap_batch(arr, sample, ef, k, threads=host["threads"])
+            ref_rows.append(dict(
+                ef=ef,
+                reference_equals_heap_replica=round(float(np.mean(np.all(h_ids == r_ids, 1))),
+                                                    5),
+                rows_identical=round(float(np.mean(np.all(ids == r_ids, 1))), 5),
+                rows_equal_up_to_ties=round(float(np.mean(same_up_to_ties(base, sample, ids,
+                                                                          r_ids))), 5)))
+    ok = all(r["ids_equal"] and r["dist_bits_equal"] for r in oracle_rows) and \
+        all(r["reference_equals_heap_replica"] == 1.0 for r in ref_rows)
+    return {"queries": int(sample.shape[0]), "vs_oracle": oracle_rows,
+            "vs_reference_search": ref_rows, "ok": ok}
+
+
+def parity_fastscan(ix, arr, sample, efs, k, host):
+    """Full-size parity of the FastScan path against the oracle's search_h2
+    on the device's own sidecar: ids and distance bits at each ef."""
+    from oracle import oracle as O
+
+    signs, codes, factors = ix.get_sidecar()
+    H = O.H2Arrays(arr, signs, codes, factors, 1)
+    rows = []
+    for ef in efs:
+        ids, dis = ix.search_batch(sample, k, ef=ef, return_distances=True)
+        e_ids, e_dis, _ = O.search_h2_batch(H, sample, ef, k, threads=host["threads"])
+        rows.append(dict(ef=ef, ids_equal=bool(np.array_equal(ids, e_ids)),
+                         dist_bits_equal=bool(np.array_equal(dis.view(np.uint32),
+                                                             e_dis.view(np.uint32)))))
+    return rows
 
 
 def read_traffic(path=TRAFFIC_FILE):
@@ -422,10 +608,12 @@ def read_traffic(path=TRAFFIC_FILE):
     return None
 
 
-def cpu_baseline(ix_path, n, d, deg, Q, ef, k, threads, budget_s=20.0):
-    """Reference search timed on the host (oracle/_ref), else the oracle port."""
+def cpu_baseline(ix_path, n, d, deg, Q, ef, k, host, nq=2000):
+    """The reference's search() (SQ8 path) timed on the host cores
+    (oracle/_ref/ref_driver), else the oracle port."""
+    threads = host["threads"]
     drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
-    sample = Q[:2000]
+    sample = Q[:nq]
     if os.path.exists(drv) and ix_path:
         with tempfile.TemporaryDirectory() as tmp:
             qp = os.path.join(tmp, "q.f32")
@@ -437,11 +625,14 @@ def cpu_baseline(ix_path, n, d, deg, Q, ef, k, threads, budget_s=20.0):
                 if r.returncode == 0:
                     res = json.loads(r.stdout.strip().splitlines()[-1])
                     return dict(value=round(res["qps_all"], 1), unit="queries/s", cores=threads,
-                                kind="reference",
+                                kind="reference", host=host,
+                                single_thread_p50_ms=res["p50_ms"],
+                                single_thread_p99_ms=res["p99_ms"],
                                 sample=f"{sample.shape[0]} queries, ef={ef}, k={k}, reference "
                                        f"search() (same traversal; its output order differs), "
-                                       f"{threads} threads each with its own index copy, median "
-                                       f"of 3 rounds; single-thread p50 {res['p50_ms']:.3f} ms "
+                                       f"{threads} threads (every host core this process may "
+                                       f"use) each with its own index copy, median of 3 rounds; "
+                                       f"single-thread p50 {res['p50_ms']:.3f} ms "
                                        f"p99 {res['p99_ms']:.3f} ms")
                 log("ref_driver failed:", r.returncode, r.stderr[-300:])
             except Exception as e:  # noqa: BLE001
@@ -453,8 +644,50 @@ def cpu_baseline(ix_path, n, d, deg, Q, ef, k, threads, budget_s=20.0):
     O.search_canon_batch(arr, sample, ef, k, "nearest", threads=threads)
     dt = time.perf_counter() - t0
     return dict(value=round(sample.shape[0] / dt, 1), unit="queries/s", cores=threads,
-                kind="port", sample=f"{sample.shape[0]} queries, ef={ef}, k={k}, C restatement, "
-                                    f"OpenMP over queries")
+                kind="port", host=host,
+                sample=f"{sample.shape[0]} queries, ef={ef}, k={k}, C restatement, "
+                       f"OpenMP over queries")
+
+
+def cpu_baseline_fastscan(ix_path, n, d, deg, signs, Q, ef, k, host, tmp, want_sidecar):
+    """The reference's search_qg (FastScan path) timed on the host cores: the
+    reference's own rows rebuilt with its rabitq_codes under the device's
+    rotation signs (oracle/_ref/ref_h2 time), all 10,000 queries, one graph
+    copy per thread, median of 3 rounds.  Also returns the reference's ids of
+    those queries and, if asked, its sidecar (codes in its block layout,
+    factors) for a search on the device with the very same codes."""
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_h2")
+    if not os.path.exists(drv):
+        return None, None, None
+    sp, qp = os.path.join(tmp, "signs.f32"), os.path.join(tmp, "qfs.f32")
+    idp, scp = os.path.join(tmp, "ref_fs_ids.u32"), os.path.join(tmp, "ref_sidecar.bin")
+    np.ascontiguousarray(signs, np.float32).tofile(sp)
+    np.ascontiguousarray(Q, np.float32).tofile(qp)
+    nq, threads = Q.shape[0], host["threads"]
+    r = subprocess.run([drv, "time", ix_path, str(n), str(d), str(deg), sp, qp, str(nq), str(ef),
+                        str(k), str(threads), "3", idp, scp if want_sidecar else "-"],
+                       capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        log("ref_h2 time failed:", r.returncode, r.stderr[-300:])
+        return None, None, None
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    ids = np.fromfile(idp, np.uint32).reshape(nq, k)
+    side = None
+    if want_sidecar:
+        p = 1 << (d - 1).bit_length()
+        raw = np.fromfile(scp, np.uint8).reshape(n, deg * p // 8 + 12 * deg)
+        side = (np.ascontiguousarray(raw[:, :deg * p // 8]),
+                np.ascontiguousarray(raw[:, deg * p // 8:]).view(np.float32))
+        os.remove(scp)
+    base = dict(value=round(res["qps_all"], 1), unit="queries/s", cores=threads,
+                kind="reference", host=host, single_thread_p50_ms=res["p50_ms"],
+                single_thread_p99_ms=res["p99_ms"],
+                sample=f"{nq} queries, ef={ef}, k={k}, the reference's search_qg on its own "
+                       f"rows rebuilt with rabitq_codes (rebuild {res['sidecar_s']:.1f} s, not "
+                       f"timed), {threads} threads (every host core this process may use) "
+                       f"each with its own graph copy, median of 3 rounds; single-thread p50 "
+                       f"{res['p50_ms']:.3f} ms p99 {res['p99_ms']:.3f} ms")
+    return base, ids, side
 
 
 def main():
@@ -469,8 +702,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fastscan", action="store_true", help="skip the FastScan RaBitQ leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving legs")
-    ap.add_argument("--all-configs", action="store_true",
-                    help="also measure the 100k x 64 and 10M x 96 workloads (about a minute)")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 100k x 64 and 10M x 96 workloads (about a minute)")
+    ap.add_argument("--no-parity", action="store_true",
+                    help="skip the full-size parity checks against the oracle and the reference")
+    ap.add_argument("--parity-nq", type=int, default=2000,
+                    help="queries of the full-size parity sample")
     ap.add_argument("--serve-n", type=int, default=1_000_000,
                     help="base vectors of the 768-d serving workload")
     ap.add_argument("--serve-nq", type=int, default=10_000,
@@ -531,6 +768,15 @@ def main():
     t = time.perf_counter()
     gt = ground_truth(base, Q, k)
     log(f"ground truth {time.perf_counter() - t:.1f}s")
+    host = host_cpus()
+    log(f"host cpus {host}")
+    tmpdir = tempfile.TemporaryDirectory()
+    ix_path = os.path.join(tmpdir.name, "c2.index")
+    ix.save(ix_path)
+    from oracle import oracle as O  # the checker: parity and CPU-baseline legs only
+
+    arr = O.read_index(ix_path, n, d, deg)
+    sample = Q[:args.parity_nq]
 
     db = DeviceBatch(ix, Q, k)
     sweep = []
@@ -558,6 +804,19 @@ def main():
     torch.cuda.synchronize()
     E, V = db.counters()
     log(f"timed ef={chosen}: {ms_step:.3f} ms/step, {qps:.0f} QPS")
+
+    # the general float SQ8 kernel on the same workload (the integer kernel
+    # above is exact only because these vectors are integers 0..255)
+    ix.set_sq8_integer(False)
+    ids_int = db.results().copy()
+    db.launch(chosen)
+    torch.cuda.synchronize()
+    float_same = bool(np.array_equal(db.results(), ids_int))
+    ms_float, _ = time_launches(db, chosen, args.steps, args.warmup)
+    ix.set_sq8_integer(True)
+    db.launch(chosen)
+    torch.cuda.synchronize()
+    log(f"float SQ8 kernel: {ms_float:.3f} ms/step, ids equal to the integer kernel: {float_same}")
 
     # the same launch captured into a HIP graph and replayed
     graph_ms = graph_leg(ix, db, chosen, args)
@@ -590,21 +849,28 @@ def main():
     achieved = alg_bytes / (ms_step / 1e3) / 1e9
     traffic = read_traffic()
 
+    parity = None
+    if not args.no_parity:
+        t = time.perf_counter()
+        parity = parity_sq8(ix, arr, base, sample, [r["ef"] for r in sweep], k, host, ix_path,
+                            tmpdir.name)
+        log(f"parity (SQ8) {parity} ({time.perf_counter() - t:.1f}s)")
+
     serving = None if args.no_serve else serving_leg(ix, Q, chosen, k, args)
 
-    fastscan = None if args.no_fastscan else fastscan_leg(ix, db, Q, gt, k, args)
-
-    serving_mix = None if args.no_serve else serving_mix_leg(args)
-
-    configs = configs_leg(args) if args.all_configs else None
+    ctx = dict(arr=arr, base=base, sample=sample, host=host, path=ix_path, tmp=tmpdir.name)
+    fastscan = None if args.no_fastscan else fastscan_leg(ix, db, Q, gt, k, args, ctx)
 
     cpu = None
     if not args.no_cpu:
-        with tempfile.TemporaryDirectory() as tmp:
-            p = os.path.join(tmp, "c2.index")
-            ix.save(p)
-            cpu = cpu_baseline(p, n, d, deg, Q, chosen, k, min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline(ix_path, n, d, deg, Q, chosen, k, host)
         log(f"cpu baseline {cpu}")
+    del arr, ctx
+    tmpdir.cleanup()
+
+    serving_mix = None if args.no_serve else serving_mix_leg(args)
+
+    configs = None if args.no_configs else configs_leg(args, host)
 
     line = {
         "metric": "QPS at recall@10≥0.95 (batched vs plain GPU), 1M×128 deg-32; p99 ms",
@@ -641,6 +907,15 @@ def main():
     }
     line["graph_replay"] = {"ms_per_step": round(graph_ms, 4),
                             "qps": round(args.nq / (graph_ms / 1e3), 1)}
+    float_gbs = alg_bytes / (ms_float / 1e3) / 1e9
+    line["sq8_float_kernel"] = {
+        "note": "the general SQ8 kernel (dequantise + FMA partials), integer kernel disabled",
+        "ms_per_step": round(ms_float, 4), "qps": round(args.nq / (ms_float / 1e3), 1),
+        "ids_equal_integer_kernel": float_same,
+        "roofline": {"bound": "hbm", "achieved": round(float_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(float_gbs / HBM_PEAK_GBS, 4)}}
+    if parity:
+        line["parity"] = parity
     # the same algorithmic bytes per batch over the back-to-back batch time
     steady_gbs = alg_bytes / (steady["ms_per_batch"] / 1e3) / 1e9
     steady.update({"algorithmic_GBps": round(steady_gbs, 1),
@@ -657,6 +932,11 @@ def main():
     if cpu:
         line["gpu_over_cpu"] = round(qps / cpu["value"], 1)
     print(json.dumps(line), flush=True)
+    bad = [name for name, p in (("SQ8", parity), ("FastScan", fastscan and fastscan["parity"]))
+           if p and not p["ok"]]
+    if bad:
+        log(f"PARITY FAILED: {bad}")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -145,8 +145,9 @@ lqg_status lqg_search_batch_device(lqg_index* index, const float* d_queries, uin
  * recorded once and replayed with one hipGraphLaunch.  The graph holds the
  * work-queue reset and the persistent search kernel (kernel nodes only), in
  * the mode and with the launch settings in force at creation.  It refers to
- * the index's device memory: destroy it before the index is reloaded, rebuilt
- * or destroyed.  No counterpart in the reference. */
+ * the index's device memory: once the index is reloaded, rebuilt or given a
+ * new sidecar, lqg_graph_launch refuses to replay (LQG_ERR_STATE); destroy
+ * the graph before the index.  No counterpart in the reference. */
 typedef struct lqg_batch_graph lqg_batch_graph;
 
 lqg_status lqg_graph_create(lqg_index* index, const float* d_queries, uint64_t nq,
@@ -190,11 +191,18 @@ lqg_status lqg_serve_start(lqg_index* index, const lqg_serve_config* cfg, lqg_se
 /* Copies the query (D floats); LQG_ERR_BUSY when every slot is taken. */
 lqg_status lqg_serve_submit(lqg_server* server, const float* query, uint32_t ef, uint32_t k,
                             uint64_t* ticket);
-/* LQG_OK and the k ids (dists optional) once done, LQG_ERR_BUSY before; the
- * ticket is released by a successful poll. */
-lqg_status lqg_serve_poll(lqg_server* server, uint64_t ticket, uint32_t* results, float* dists);
+/* LQG_OK and the request's k ids (dists optional) once done, LQG_ERR_BUSY
+ * before.  capacity = entries results and dists hold: below the request's k
+ * the call fails with LQG_ERR_ARG and the ticket stays valid.  A request that
+ * could not finish (its launch failed, or its own FastScan visited set
+ * overflowed; the other requests of its batch are unaffected) returns
+ * LQG_ERR_DEVICE.  The ticket is released by a poll that returns LQG_OK or
+ * LQG_ERR_DEVICE. */
+lqg_status lqg_serve_poll(lqg_server* server, uint64_t ticket, uint32_t* results, float* dists,
+                          uint32_t capacity);
 /* Blocks until done, then as lqg_serve_poll. */
-lqg_status lqg_serve_wait(lqg_server* server, uint64_t ticket, uint32_t* results, float* dists);
+lqg_status lqg_serve_wait(lqg_server* server, uint64_t ticket, uint32_t* results, float* dists,
+                          uint32_t capacity);
 lqg_status lqg_serve_get_stats(lqg_server* server, lqg_serve_stats* stats);
 /* Drains in-flight work and frees the server. */
 lqg_status lqg_serve_stop(lqg_server* server);
@@ -215,10 +223,13 @@ typedef struct {
     int32_t waves_per_group; /* wavefronts (queries in flight) per workgroup, 1..4      */
     int32_t cache_bits;      /* log2 entries of the per-wave visited cache, 4..14 (SQ8
                                 mode); of the LDS visited table, 6..14 (FastScan)    */
-    int32_t occupancy;       /* kernel register cap, 0 = uncapped.  SQ8: 8 = 64 VGPRs
+    int32_t occupancy;       /* kernel register cap.  SQ8: 0 = default (at least 4
+                                waves/SIMD, 128 VGPRs, for pool classes up to 256;
+                                uncapped above), -1 = uncapped, 8 = 64 VGPRs
                                 (8 waves/SIMD), 6 = 80 VGPRs (6 waves/SIMD).
-                                FastScan: 5 or 6 waves/SIMD, for indexes with
-                                P / lanes per neighbour <= 128 code bits        */
+                                FastScan: 0 = uncapped, 5 or 6 waves/SIMD, for
+                                indexes with P / lanes per neighbour <= 128 code
+                                bits                                             */
     int32_t groups;          /* workgroups of the persistent grid; 0 = fill the device  */
 } lqg_launch;
 
@@ -242,6 +253,15 @@ lqg_status lqg_autotune(lqg_index* index, const float* queries, uint64_t nq, uin
  * (qg.hpp:320-328).  Used for algorithmic-byte accounting; not for speed. */
 lqg_status lqg_count_visits(lqg_index* index, const float* queries, uint64_t nq, uint32_t ef,
                             uint32_t k, uint32_t* stats);
+
+/* Diagnostic: enable (default) or disable the exact integer SQ8 kernel.  An
+ * index whose SQ8 parameters are all min 0 / scale 1, searched with queries
+ * integer valued in [0, 255] (D % 16 == 0, D <= 256), has every FP32 step of
+ * the reference's distance exact, and a kernel computing the distances with
+ * integer dot products gives bit-identical results.  Disabled, every query
+ * runs the general float kernel (dequantise, FMA partials).  Results do not
+ * change; only speed does.  No counterpart in the reference. */
+lqg_status lqg_set_sq8_integer(lqg_index* index, int enable);
 
 /* Selects the search path.  LQG_MODE_FASTSCAN requires a sidecar.  No
  * counterpart in the reference, whose search() is hard-wired to the SQ8 path
@@ -298,6 +318,16 @@ lqg_status lqg_h2_rotate(lqg_index* index, const float* queries, uint64_t nq,
 lqg_status lqg_h2_probe(lqg_index* index, const float* queries, uint64_t nq,
                         const uint32_t* nodes, uint8_t* lut, float* lo_width, int32_t* sumq,
                         float* sqr_y, float* est);
+
+/* FastScan visited set: each query's exact set lives in an LDS table sized by
+ * its pool class and moves, past half full, to a per-wave table of 2^bits
+ * entries in device memory (default 14); past half of that the query fails
+ * (E = 0xFFFFFFFF in d_stats, ids 0xFFFFFFFF, its own serving request fails).
+ * bits = 0: no device table, a query fails once its LDS table is half full
+ * (for tests of that failure path); otherwise 8 <= bits <= 16.  No
+ * counterpart in the reference, whose hash set grows without bound
+ * (third/ngt/hashset.hpp:30-110). */
+lqg_status lqg_h2_set_overflow_bits(lqg_index* index, uint32_t bits);
 
 /* Read-only views of the loaded index (host copies). */
 lqg_status lqg_info(const lqg_index* index, uint64_t* num_elements, uint64_t* dimension,

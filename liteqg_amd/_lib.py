@@ -69,6 +69,7 @@ _SIGS = {
     "lqg_count_visits": [_P, _P, C.c_uint64, C.c_uint32, C.c_uint32, _P],
     "lqg_info": [_P, _P, _P, _P, _P, _P],
     "lqg_get_graph": [_P, _P],
+    "lqg_set_sq8_integer": [_P, C.c_int],
     "lqg_set_mode": [_P, C.c_int],
     "lqg_get_mode": [_P, _P],
     "lqg_h2_build_sidecar": [_P, _P, C.c_uint64],
@@ -80,14 +81,15 @@ _SIGS = {
     "lqg_h2_get_rotation": [_P, _P],
     "lqg_h2_rotate": [_P, _P, C.c_uint64, C.c_int, _P],
     "lqg_h2_probe": [_P, _P, C.c_uint64, _P, _P, _P, _P, _P, _P],
+    "lqg_h2_set_overflow_bits": [_P, C.c_uint32],
     "lqg_graph_create": [_P, _P, C.c_uint64, _P, _P, C.c_uint32, C.c_uint32, C.c_int, _P, _P, _P,
                          C.POINTER(_P)],
     "lqg_graph_launch": [_P, _P],
     "lqg_graph_destroy": [_P],
     "lqg_serve_start": [_P, _P, C.POINTER(_P)],
     "lqg_serve_submit": [_P, _P, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)],
-    "lqg_serve_poll": [_P, C.c_uint64, _P, _P],
-    "lqg_serve_wait": [_P, C.c_uint64, _P, _P],
+    "lqg_serve_poll": [_P, C.c_uint64, _P, _P, C.c_uint32],
+    "lqg_serve_wait": [_P, C.c_uint64, _P, _P, C.c_uint32],
     "lqg_serve_get_stats": [_P, _P],
     "lqg_serve_stop": [_P],
     "lqg_serve_replay": [_P, _P, C.c_uint64, _P, _P, _P, _P, _P],

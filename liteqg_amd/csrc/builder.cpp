@@ -160,6 +160,7 @@ lqg_status build_index(lqg_index* h, const float* data, uint32_t ef_build, uint3
     random_graph(n, g, h->nbr.data());
     h->have_data = true;
     h->on_device = false;
+    ++h->generation;
     lqg_status st = ensure_device(h);
     if (st != LQG_OK) return st;
 
@@ -256,6 +257,7 @@ lqg_status build_index(lqg_index* h, const float* data, uint32_t ef_build, uint3
     if (rs != LQG_OK) {
         h->have_data = false;
         h->on_device = false;
+        ++h->generation;
     }
     return rs;
 }

@@ -616,10 +616,13 @@ __global__ void reset_counter_kernel(uint32_t* counter, uint32_t n) {
 
 using KernelFn = void (*)(DevIndex, SearchJob, uint32_t, uint32_t, uint32_t);
 
+// occ: 8 or 6 waves per SIMD (64 / 80 VGPRs); -1 uncapped; 0 the default, at
+// least 4 waves per SIMD (128 VGPRs) up to the 256 class and uncapped above.
 template <int EF_CAP, int EVAL>
 KernelFn kernel_for(int occ) {
     if (occ == 8) return &h1_search_kernel_w8<EF_CAP, EVAL>;
     if (occ == 6) return &h1_search_kernel_w6<EF_CAP, EVAL>;
+    if (occ == -1) return &h1_search_kernel<EF_CAP, EVAL>;
     if constexpr (EF_CAP <= 256)
         return &h1_search_kernel_w4<EF_CAP, EVAL>;
     else

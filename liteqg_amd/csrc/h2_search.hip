@@ -708,13 +708,14 @@ __device__ void h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
         }
     }
 
-    // copy_results (buffer.hpp:102): unfilled slots hold id 0
+    // copy_results (buffer.hpp:102): unfilled slots hold id 0; a query whose
+    // visited set overflowed returns no ids (0xFFFFFFFF everywhere)
     uint32_t* oi = job.out_ids + static_cast<size_t>(q) * job.kstride;
     float* od = job.out_d ? job.out_d + static_cast<size_t>(q) * job.kstride : nullptr;
     for (uint32_t j = lane; j < job.kstride; j += kWave) {
         uint32_t id = kInvalid;
         float d = INFINITY;
-        if (j < kq) {
+        if (j < kq && !vis.failed) {
             id = j < rsize ? w.ri[j] : 0u;
             if (j < rsize) d = w.rd[j];
         }
@@ -728,8 +729,7 @@ __device__ void h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
         }
         if (vis.failed) atomicOr(job.err, 1u);
     }
-    // a query whose visited set overflowed gets no flag: its batch fails as a whole
-    if (job.done && !vis.failed) signal_host(job.done + q, lane);
+    if (job.done) signal_host(job.done + q, lane, vis.failed ? kFlagFailed : kFlagDone);
     wave_sync();
 }
 

@@ -106,6 +106,7 @@ lqg_status ensure_device(lqg_index* h) {
     LQG_HIP_TRY(hipMemcpy(h->d_scale.p, sc.data(), h->dim * 4, hipMemcpyHostToDevice));
     LQG_HIP_TRY(h->scratch.counter.reserve(1));
     h->on_device = true;
+    ++h->generation;
     return LQG_OK;
 }
 
@@ -167,7 +168,8 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
     // 256) runs the integer-path kernel first; a float-path launch then takes
     // only the queries that were not integer valued.  Each kernel variant
     // holds only its own distance code's registers.
-    const bool int_index = h->sq8_identity && h->dim % 16 == 0 && h->dim <= 256;
+    const bool int_index =
+        h->sq8_integer && h->sq8_identity && h->dim % 16 == 0 && h->dim <= 256;
     // rows of at most 128 bytes take the integer variant with half the row buffer
     const int int_eval = h->dim <= 128 ? 3 : 1;
     if (count_visits) {
@@ -357,6 +359,7 @@ lqg_status lqg_load(lqg_index* h, const char* filename) {
     h->sq8.swap(sq8);
     h->have_data = true;
     h->on_device = false;
+    ++h->generation;
     lqg::h2_reset(h);
     return LQG_OK;
 }
@@ -393,6 +396,12 @@ lqg_status lqg_set_order(lqg_index* h, lqg_order order) {
     if (!h) return fail(LQG_ERR_ARG, "null index");
     if (!valid_order(order)) return fail(LQG_ERR_ARG, "unknown order");
     h->order = order;
+    return LQG_OK;
+}
+
+lqg_status lqg_set_sq8_integer(lqg_index* h, int enable) {
+    if (!h) return fail(LQG_ERR_ARG, "null index");
+    h->sq8_integer = enable != 0;
     return LQG_OK;
 }
 
@@ -447,9 +456,12 @@ lqg_status lqg_search(lqg_index* h, const float* query, uint32_t k, uint32_t* re
     // wait for the flag; a stream that finishes without it had a query fail
     // (a FastScan visited set that overflowed) or a launch error
     for (uint32_t spin = 1;; ++spin) {
-        if (__atomic_load_n(&h->p_out[0], __ATOMIC_ACQUIRE) != 0u) {
-            memcpy(results, h->p_out + 16, 4ull * k);
+        const uint32_t flag = __atomic_load_n(&h->p_out[0], __ATOMIC_ACQUIRE);
+        if (flag != 0u) {
             HIP_TRY(hipStreamSynchronize(h->stream));
+            if (flag != lqg::kFlagDone)
+                return fail(LQG_ERR_UNSUPPORTED, "FastScan: the query's visited set overflowed");
+            memcpy(results, h->p_out + 16, 4ull * k);
             return LQG_OK;
         }
         if ((spin & 255u) == 0u) {
@@ -560,8 +572,9 @@ lqg_status lqg_set_launch(lqg_index* h, uint32_t ef, const lqg_launch* cfg) {
     const bool fs = h->mode == LQG_MODE_FASTSCAN;
     if (fs && cfg && cfg->cache_bits < 6)
         return fail(LQG_ERR_ARG, "FastScan visited-table bits must be in [6, 14]");
-    if (cfg && !fs && cfg->occupancy != 0 && cfg->occupancy != 6 && cfg->occupancy != 8)
-        return fail(LQG_ERR_ARG, "SQ8 occupancy must be 0, 6 or 8");
+    if (cfg && !fs && cfg->occupancy != -1 && cfg->occupancy != 0 && cfg->occupancy != 6 &&
+        cfg->occupancy != 8)
+        return fail(LQG_ERR_ARG, "SQ8 occupancy must be -1, 0, 6 or 8");
     if (cfg && fs && cfg->occupancy != 0 && cfg->occupancy != 5 && cfg->occupancy != 6)
         return fail(LQG_ERR_ARG, "FastScan occupancy must be 0, 5 or 6");
     if (cfg && fs && cfg->occupancy != 0 && h->h2_ready && !lqg::h2_occupancy_ok(h->h2dev()))
@@ -699,14 +712,16 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
     const lqg::LaunchCfg saved = h->launch[slot];
     float best_ms = 1e30f;
     lqg::LaunchCfg best_cfg = lqg::default_launch(cls);
-    const int occs[] = {0, 6, 8};
+    // -1 (uncapped) differs from the default only up to the 256 class
+    const int occs[] = {0, 6, 8, -1};
+    const int n_occ = cls <= 256 ? 4 : 3;
     const int bits[] = {8, 9, 10, 11};
     const int wpg[] = {1, 2, 4};
-    for (int occ : occs)
+    for (int oi = 0; oi < n_occ; ++oi)
         for (int cb : bits)
             for (int wg : wpg) {
                 lqg::LaunchCfg c = lqg::default_launch(cls);
-                c.occupancy = occ;
+                c.occupancy = occs[oi];
                 c.cache_bits = cb;
                 c.waves_per_group = wg;
                 h->launch[slot] = c;

@@ -7,6 +7,7 @@
 struct lqg_batch_graph {
     lqg_index* index = nullptr;
     lqg::Scratch scratch;  // its own work-queue counter and overflow tables
+    uint64_t generation = 0;  // the index's generation at capture
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -65,12 +66,16 @@ lqg_status lqg_graph_create(lqg_index* h, const float* d_queries, uint64_t nq,
         return st != LQG_OK ? st : lqg::hip_fail(e, "graph capture");
     }
     g->index = h;
+    g->generation = h->generation;
     *out = g;
     return LQG_OK;
 }
 
 lqg_status lqg_graph_launch(lqg_batch_graph* g, void* stream) {
     if (!g || !g->exec) return fail(LQG_ERR_ARG, "null graph");
+    if (g->generation != g->index->generation)
+        return fail(LQG_ERR_STATE, "the index was reloaded, rebuilt or given a new sidecar since "
+                                   "this graph was captured: destroy it and capture again");
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : g->index->stream;
     LQG_HIP_TRY(hipGraphLaunch(g->exec, s));
     return LQG_OK;

@@ -18,10 +18,6 @@ namespace lqg {
 
 namespace {
 
-// Per-wave overflow visited table in device memory: 2^14 entries (64 KiB),
-// used once a query has expanded more nodes than half its LDS table holds.
-constexpr uint32_t kGvisBits = 14;
-
 uint32_t padded_dim(uint64_t d) {
     uint32_t p = 1;
     while (p < d) p <<= 1;
@@ -79,6 +75,7 @@ lqg_status alloc_records(lqg_index* h) {
     lqg_status st = set_layout(h);
     if (st != LQG_OK) return st;
     h->h2_ready = false;
+    ++h->generation;
     LQG_HIP_TRY(h->d_h2rec.reserve(h->n * static_cast<size_t>(h->h2_stride)));
     LQG_HIP_TRY(h->d_h2signs.reserve(h->h2_p));
     st = host_rows(h, 0, h->raw.data(), 4 * h->dim, true);
@@ -170,6 +167,7 @@ H2Cfg h2_effective_launch(const lqg_index* h, int ef_cap) {
 }
 
 void h2_reset(lqg_index* h) {
+    ++h->generation;
     h->d_h2rec.release();
     h->h2_ready = false;
     h->mode = LQG_MODE_SQ8;
@@ -191,7 +189,11 @@ lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const
     const int groups = h2_groups(ix, cfg, kmax, static_cast<uint32_t>(nq));
     if (groups <= 0) return fail(LQG_ERR_DEVICE, "FastScan: no launch geometry");
     cfg.groups = groups;
-    LQG_HIP_TRY(sc->gvis.reserve((static_cast<size_t>(groups) * cfg.waves_per_group) << kGvisBits));
+    // per-wave overflow visited tables in device memory (2^h2_gvis_bits
+    // entries each), used once a query has expanded more nodes than half its
+    // LDS table holds; none when h2_gvis_bits is 0
+    const uint32_t gbits = h->h2_gvis_bits;
+    if (gbits) LQG_HIP_TRY(sc->gvis.reserve((static_cast<size_t>(groups) * cfg.waves_per_group) << gbits));
     LQG_HIP_TRY(sc->err.reserve(1));
     LQG_HIP_TRY(sc->counter.reserve(1));
     if (h->h2_rotation == LQG_ROTATION_GEMM) {
@@ -211,8 +213,8 @@ lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const
     job.out_d = d_dists;
     job.out_stats = d_stats;
     job.counter = sc->counter.p;
-    job.gvis = sc->gvis.p;
-    job.gvis_bits = kGvisBits;
+    job.gvis = gbits ? sc->gvis.p : nullptr;
+    job.gvis_bits = gbits;
     job.err = sc->err.p;
     job.done = sc->done;
     job.rot = h->h2_rotation == LQG_ROTATION_GEMM ? sc->rot.p : nullptr;
@@ -227,7 +229,10 @@ lqg_status h2_check_error(lqg_index* h, Scratch* sc) {
     if (h->mode != LQG_MODE_FASTSCAN || !sc->err.p) return LQG_OK;
     uint32_t e = 0;
     LQG_HIP_TRY(hipMemcpy(&e, sc->err.p, 4, hipMemcpyDeviceToHost));
-    if (e) return fail(LQG_ERR_UNSUPPORTED, "FastScan: a query's visited set overflowed");
+    if (e)
+        return fail(LQG_ERR_UNSUPPORTED,
+                    "FastScan: a query's visited set overflowed (its ids are 0xFFFFFFFF; the "
+                    "other queries' results are complete)");
     return LQG_OK;
 }
 
@@ -447,6 +452,14 @@ lqg_status lqg_h2_load_sidecar(lqg_index* h, const char* path) {
     fclose(f);
     if (!ok) return fail(LQG_ERR_IO, std::string("short read from ") + path);
     h->h2_ready = true;
+    return LQG_OK;
+}
+
+lqg_status lqg_h2_set_overflow_bits(lqg_index* h, uint32_t bits) {
+    if (!h) return fail(LQG_ERR_ARG, "null index");
+    if (bits != 0 && (bits < 8 || bits > 16))
+        return fail(LQG_ERR_ARG, "overflow table bits must be 0 or in [8, 16]");
+    h->h2_gvis_bits = bits;
     return LQG_OK;
 }
 

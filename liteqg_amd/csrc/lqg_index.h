@@ -91,9 +91,15 @@ struct lqg_index {
     std::vector<uint8_t> qdata;  // n x dim
     std::vector<float> sq8;      // dim x {min, scale}
 
+    // Bumped whenever device memory the kernels read may be freed, moved or
+    // rewritten (upload, reload, rebuild, sidecar replaced): a captured graph
+    // holds those addresses and refuses to replay across a change.
+    uint64_t generation = 0;
+
     // device copy
     bool on_device = false;
     bool sq8_identity = false;  // all SQ8 params are min 0, scale 1
+    bool sq8_integer = true;    // the exact integer kernel may be used (lqg_set_sq8_integer)
     hipStream_t stream = nullptr;
     lqg::DevBuf<uint8_t> d_qdata;
     lqg::DevBuf<uint32_t> d_nbr;
@@ -128,6 +134,7 @@ struct lqg_index {
     lqg::H2Cfg h2_launch[lqg::kClasses];
     bool h2_launch_set[lqg::kClasses] = {};
     lqg_rotation h2_rotation = LQG_ROTATION_FHT;
+    uint32_t h2_gvis_bits = 14;  // per-wave overflow visited table, log2 entries (0: none)
 
     lqg::H2Dev h2dev() const {
         lqg::H2Dev ix;

@@ -9,6 +9,11 @@ namespace lqg {
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 constexpr int kWave = 64;
 
+// Completion flag values a launch writes per query (host-coherent memory,
+// SearchJob::done / H2Job::done).
+constexpr uint32_t kFlagDone = 1u;    // results written
+constexpr uint32_t kFlagFailed = 2u;  // the query could not finish (ids 0xFFFFFFFF)
+
 // Device view of an index for the SQ8 search (the reference's live path,
 // qg.hpp:298-344).  qdata rows are `dim` bytes, neighbour rows `deg` ids.
 struct DevIndex {

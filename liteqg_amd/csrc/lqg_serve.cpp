@@ -56,12 +56,10 @@ struct Lane {
     uint32_t* g_ids = nullptr;  // the same three as device addresses
     float* g_dist = nullptr;
     uint32_t* g_flag = nullptr;
-    uint32_t* h_err = nullptr;  // pinned: FastScan overflow flag
     std::vector<uint32_t> slots;
     std::vector<uint8_t> out;   // request i of the batch already handed back
     size_t n_out = 0;
     bool busy = false;
-    bool fastscan = false;
     bool failed = false;
 };
 
@@ -100,7 +98,6 @@ void free_lane(Lane& l) {
     if (l.h_ids) (void)hipHostFree(l.h_ids);
     if (l.h_dist) (void)hipHostFree(l.h_dist);
     if (l.h_flag) (void)hipHostFree(l.h_flag);
-    if (l.h_err) (void)hipHostFree(l.h_err);
     if (l.done) (void)hipEventDestroy(l.done);
     if (l.stream) (void)hipStreamDestroy(l.stream);
     l = Lane();
@@ -150,7 +147,6 @@ void launch(lqg_server* s, Lane& l, const std::vector<uint32_t>& batch) {
     memset(l.h_flag, 0, 4ull * b);
     l.scratch.done = l.g_flag;
     l.failed = false;
-    l.fastscan = s->h->mode == LQG_MODE_FASTSCAN;
     bool ok = hipMemcpyAsync(l.d_q.p, l.h_q, 4ull * b * d, hipMemcpyHostToDevice, l.stream) ==
                   hipSuccess &&
               hipMemcpyAsync(l.d_ef.p, l.h_ek, 4ull * b, hipMemcpyHostToDevice, l.stream) ==
@@ -160,9 +156,6 @@ void launch(lqg_server* s, Lane& l, const std::vector<uint32_t>& batch) {
     ok = ok && lqg::run_batch(s->h, l.d_q.p, b, l.d_ef.p, l.d_k.p, s->cfg.ef_max, s->cfg.kmax,
                               s->cfg.order, l.g_ids, l.g_dist, nullptr, l.stream, false,
                               &l.scratch) == LQG_OK;
-    if (ok && l.fastscan)
-        ok = hipMemcpyAsync(l.h_err, l.scratch.err.p, 4, hipMemcpyDeviceToHost, l.stream) ==
-             hipSuccess;
     ok = ok && hipEventRecord(l.done, l.stream) == hipSuccess;
     l.failed = !ok;
     l.busy = true;
@@ -184,7 +177,9 @@ void hand_back(lqg_server* s, Lane& l, size_t i, bool bad, int64_t t) {
 }
 
 // Hands back the requests of a running lane whose flags the kernels have set
-// (their results were written before the flag).  True if any.
+// (their results were written before the flag; kFlagFailed marks a request
+// that could not finish, a FastScan visited set that overflowed).  True if
+// any.
 bool deliver(lqg_server* s, Lane& l, std::vector<uint32_t>& ready) {
     if (l.failed || l.n_out == l.slots.size()) return false;
     ready.clear();
@@ -195,24 +190,29 @@ bool deliver(lqg_server* s, Lane& l, std::vector<uint32_t>& ready) {
     const int64_t t = now_ns();
     {
         std::lock_guard<std::mutex> g(s->mu);
-        for (uint32_t i : ready) hand_back(s, l, i, false, t);
+        for (uint32_t i : ready)
+            hand_back(s, l, i, __atomic_load_n(&l.h_flag[i], __ATOMIC_ACQUIRE) != lqg::kFlagDone,
+                      t);
     }
     s->cv_done.notify_all();
     return true;
 }
 
-// The lane's launch has completed: hands back what is left (all of it failed
-// if the launch did, or if a FastScan visited set overflowed) and frees it.
+// The lane's launch has completed: hands back what is left and frees the
+// lane.  Every request whose flag is set is handed back by its flag (done or
+// failed on its own); a request without a flag failed with its launch.
 void retire(lqg_server* s, Lane& l) {
     // a launch that failed to enqueue may still have copies in flight that
     // read the lane's staging buffers
     if (l.failed) (void)hipStreamSynchronize(l.stream);
-    const bool bad = l.failed || (l.fastscan && l.h_err[0] != 0);
     const int64_t t = now_ns();
     {
         std::lock_guard<std::mutex> g(s->mu);
-        for (size_t i = 0; i < l.slots.size(); ++i)
-            if (!l.out[i]) hand_back(s, l, i, bad, t);
+        for (size_t i = 0; i < l.slots.size(); ++i) {
+            if (l.out[i]) continue;
+            const uint32_t f = __atomic_load_n(&l.h_flag[i], __ATOMIC_ACQUIRE);
+            hand_back(s, l, i, f != lqg::kFlagDone, t);
+        }
     }
     l.slots.clear();
     l.busy = false;
@@ -269,13 +269,17 @@ void dispatcher(lqg_server* s) {
     }
 }
 
+// capacity: entries results / dists hold; below the request's k the call
+// fails (LQG_ERR_ARG) and the ticket stays valid.
 lqg_status poll_impl(lqg_server* s, uint64_t ticket, uint32_t* results, float* dists,
-                     bool block, int64_t* t_done) {
+                     uint32_t capacity, bool block, int64_t* t_done) {
     const uint32_t slot = static_cast<uint32_t>(ticket & 0xFFFFFFFFu);
     const uint32_t g = static_cast<uint32_t>(ticket >> 32);
     std::unique_lock<std::mutex> lk(s->mu);
     if (slot >= s->cfg.slots || s->gen[slot] != g || s->state[slot].load() == kFree)
         return fail(LQG_ERR_ARG, "unknown or released ticket");
+    if ((results || dists) && capacity < s->k[slot])
+        return fail(LQG_ERR_ARG, "result buffer smaller than the request's k");
     for (;;) {
         const int st = s->state[slot].load(std::memory_order_acquire);
         if (st == kDone || st == kFailed) {
@@ -288,8 +292,8 @@ lqg_status poll_impl(lqg_server* s, uint64_t ticket, uint32_t* results, float* d
             s->state[slot].store(kFree, std::memory_order_relaxed);
             s->free_list.push_back(slot);
             return st == kDone ? LQG_OK
-                               : fail(LQG_ERR_DEVICE, "serving launch failed (or a FastScan "
-                                                      "visited set overflowed)");
+                               : fail(LQG_ERR_DEVICE, "request failed: its launch failed, or "
+                                                      "its FastScan visited set overflowed");
         }
         if (!block) return LQG_ERR_BUSY;
         s->cv_done.wait(lk);
@@ -409,8 +413,6 @@ lqg_status lqg_serve_start(lqg_index* h, const lqg_serve_config* cfg, lqg_server
             e = hipHostGetDevicePointer(reinterpret_cast<void**>(&l.g_dist), l.h_dist, 0);
         if (e == hipSuccess)
             e = hipHostGetDevicePointer(reinterpret_cast<void**>(&l.g_flag), l.h_flag, 0);
-        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&l.h_err), 4);
-        if (e == hipSuccess) l.h_err[0] = 0;
     }
     if (e != hipSuccess) {
         destroy(s);
@@ -434,14 +436,16 @@ lqg_status lqg_serve_submit(lqg_server* s, const float* query, uint32_t ef, uint
     return submit_impl(s, query, ef, k, ticket);
 }
 
-lqg_status lqg_serve_poll(lqg_server* s, uint64_t ticket, uint32_t* results, float* dists) {
+lqg_status lqg_serve_poll(lqg_server* s, uint64_t ticket, uint32_t* results, float* dists,
+                          uint32_t capacity) {
     if (!s) return fail(LQG_ERR_ARG, "null server");
-    return poll_impl(s, ticket, results, dists, false, nullptr);
+    return poll_impl(s, ticket, results, dists, capacity, false, nullptr);
 }
 
-lqg_status lqg_serve_wait(lqg_server* s, uint64_t ticket, uint32_t* results, float* dists) {
+lqg_status lqg_serve_wait(lqg_server* s, uint64_t ticket, uint32_t* results, float* dists,
+                          uint32_t capacity) {
     if (!s) return fail(LQG_ERR_ARG, "null server");
-    return poll_impl(s, ticket, results, dists, true, nullptr);
+    return poll_impl(s, ticket, results, dists, capacity, true, nullptr);
 }
 
 lqg_status lqg_serve_get_stats(lqg_server* s, lqg_serve_stats* out) {
@@ -502,7 +506,7 @@ lqg_status lqg_serve_replay(lqg_server* s, const float* queries, uint64_t nq, co
             }
             int64_t td = 0;
             uint32_t* out = results ? results + i * km : buf.data();
-            if (poll_impl(s, tickets[i], out, nullptr, true, &td) != LQG_OK) failed.store(1);
+            if (poll_impl(s, tickets[i], out, nullptr, km, true, &td) != LQG_OK) failed.store(1);
             if (results)
                 for (uint32_t j = k[i]; j < km; ++j) out[j] = 0xFFFFFFFFu;
             const int64_t sched = t0 + static_cast<int64_t>(arrival_ns[i]);

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "lqg_internal.h"
+
 namespace lqg {
 
 // Orders LDS accesses of the lanes of one wavefront (no workgroup barrier).
@@ -40,11 +42,12 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
 }
 
-// Sets flag to 1 once every lane's earlier writes are visible to the host
-// (system-scope fence on every lane, then one store).
-__device__ __forceinline__ void signal_host(uint32_t* flag, uint32_t lane) {
+// Sets flag to `value` once every lane's earlier writes are visible to the
+// host (system-scope fence on every lane, then one store).
+__device__ __forceinline__ void signal_host(uint32_t* flag, uint32_t lane,
+                                            uint32_t value = kFlagDone) {
     __threadfence_system();
-    if (lane == 0) __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (lane == 0) __hip_atomic_store(flag, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 }  // namespace lqg

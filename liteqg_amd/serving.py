@@ -47,10 +47,13 @@ class Server:
         return t.value
 
     def poll(self, ticket: int, k: int):
-        """(ids, dists) once done, else None.  A successful poll releases the ticket."""
+        """(ids, dists) once done, else None.  k is the request's k (the size
+        of the arrays returned); a k below it raises and keeps the ticket.  A
+        poll that returns or raises for a failed request releases the
+        ticket."""
         ids = np.empty(int(k), np.uint32)
         dis = np.empty(int(k), np.float32)
-        st = lib().lqg_serve_poll(self._s, C.c_uint64(ticket), ptr(ids), ptr(dis))
+        st = lib().lqg_serve_poll(self._s, C.c_uint64(ticket), ptr(ids), ptr(dis), int(k))
         if st == LQG_ERR_BUSY:
             return None
         check(st)
@@ -59,7 +62,7 @@ class Server:
     def wait(self, ticket: int, k: int):
         ids = np.empty(int(k), np.uint32)
         dis = np.empty(int(k), np.float32)
-        check(lib().lqg_serve_wait(self._s, C.c_uint64(ticket), ptr(ids), ptr(dis)))
+        check(lib().lqg_serve_wait(self._s, C.c_uint64(ticket), ptr(ids), ptr(dis), int(k)))
         return ids, dis
 
     def stats(self) -> dict:

@@ -168,7 +168,18 @@ class Index:
         check(lib().lqg_count_visits(self._h, ptr(Q), Q.shape[0], int(ef), int(k), ptr(out)))
         return out
 
+    def set_sq8_integer(self, enable: bool) -> None:
+        """Diagnostic: allow (default) or forbid the exact integer SQ8 kernel;
+        forbidden, every query runs the general float kernel (same results)."""
+        check(lib().lqg_set_sq8_integer(self._h, 1 if enable else 0))
+
     # -- FastScan RaBitQ path (search_qg) ----------------------------------
+    def set_overflow_bits(self, bits: int) -> None:
+        """log2 entries of the per-wave device visited table a FastScan query
+        moves to once its LDS table is half full (default 14); 0 = none, the
+        query fails instead (diagnostic)."""
+        check(lib().lqg_h2_set_overflow_bits(self._h, int(bits)))
+
     @property
     def padded_dim(self) -> int:
         p = 1

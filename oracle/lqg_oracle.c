@@ -17,7 +17,6 @@
  *                          ids) and against invariants (pool = ef smallest keys
  *                          of everything evaluated, every pool entry expanded).
  *   orc_bruteforce_fp64 -- against numpy argsort of FP64 distances.
- *   orc_knn_exact_f32   -- build-side helper, checked like the brute force.
  *
  * H2 (FastScan RaBitQ search_qg, qg.hpp:252-287), tests/test_oracle_h2.py:
  *   orc_buffers_replay  -- SearchBuffer/ResultBuffer against the reference's

@@ -22,14 +22,28 @@
 //     out_prep: for the first NPREP queries: rotated query (P f32), lower,
 //     width (f32), sumq (i32), LUT (4P u8), then for the entry point's row the
 //     exact distance sqr_y (f32) and the deg estimates (f32).
+//   time <index> N D deg <signs.f32 (P)> <queries.f32> NQ ef k threads rounds
+//        <out_ids.u32 | -> <out_sidecar.bin | ->
+//     the same rows, then the reference's search_qg timed as the FastScan CPU
+//     baseline: single-thread per-query latency p50/p99 over the NQ queries,
+//     and all-thread QPS with one QuantizedGraph copy per thread (search
+//     scratch is per object, qg.hpp:66-71), the median of `rounds` passes
+//     after a warm-up pass.  Prints one JSON line.  out_ids (NQ x k) are the
+//     ids of the single-thread pass, out_sidecar the rebuilt rows' codes and
+//     factors as `search` writes them ("-" skips either).
 //   buffers <ops.u32> NOPS cap <out.bin>
 //     replays (kind, id, dist-bits) ops on a SearchBuffer(cap) and a
 //     ResultBuffer(cap) (utils/buffer.hpp) and dumps the sequences of popped
 //     ids and the final ResultBuffer ids.
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -56,6 +70,7 @@ class QGBuilder {
             std::memcpy(dst, src, d * sizeof(float));
             std::memcpy(dst + nb_off, src + g.neighbor_offset_, deg * sizeof(float));
         }
+#pragma omp parallel for schedule(dynamic, 256)
         for (size_t i = 0; i < n; ++i) {
             float* dst = &nd.at(row * i);
             const PID* nb = reinterpret_cast<const PID*>(dst + nb_off);
@@ -71,6 +86,19 @@ class QGBuilder {
         g.factor_offset_ = fac_off;
         g.neighbor_offset_ = nb_off;
         g.row_offset_ = row;
+    }
+
+    // A second graph with the same rows (a copy of the data array and the
+    // offsets), for one search scratch per thread.
+    static void clone_rows(const QuantizedGraph& g, QuantizedGraph& o) {
+        decltype(g.data_) nd(std::vector<size_t>{g.num_points_, g.row_offset_});
+        std::memcpy(&nd.at(0), &g.data_.at(0), g.num_points_ * g.row_offset_ * sizeof(float));
+        o.data_ = std::move(nd);
+        o.entry_point_ = g.entry_point_;
+        o.code_offset_ = g.code_offset_;
+        o.factor_offset_ = g.factor_offset_;
+        o.neighbor_offset_ = g.neighbor_offset_;
+        o.row_offset_ = g.row_offset_;
     }
 
     static void load_signs(QuantizedGraph& g, const char* path) {
@@ -159,6 +187,78 @@ int cmd_search(int argc, char** argv) {
     return 0;
 }
 
+int cmd_time(int argc, char** argv) {
+    if (argc != 15) return 1;
+    const size_t n = arg(argv, 3), d = arg(argv, 4), deg = arg(argv, 5);
+    const size_t nq = arg(argv, 8), ef = arg(argv, 9), k = arg(argv, 10);
+    const int threads = static_cast<int>(arg(argv, 11));
+    const size_t rounds = arg(argv, 12);
+    if (threads < 1 || nq == 0 || rounds == 0) return 1;
+    using clk = std::chrono::steady_clock;
+    const auto t_load = clk::now();
+    symqg::QuantizedGraph g(n, deg, d);
+    g.load_index(argv[2]);
+    symqg::QGBuilder::load_signs(g, argv[6]);
+    symqg::QGBuilder::add_sidecar(g);
+    g.set_ef(ef);
+    const double side_s = std::chrono::duration<double>(clk::now() - t_load).count();
+    auto q = read_raw<float>(argv[7], nq * d);
+
+    // single thread: a warm-up pass (its ids are the output), then latency
+    std::vector<uint32_t> out(nq * k, 0xFFFFFFFFu);
+    for (size_t i = 0; i < nq; ++i)
+        symqg::QGBuilder::search_h2(g, &q[i * d], static_cast<uint32_t>(k), &out[i * k]);
+    std::vector<double> lat(nq);
+    std::vector<uint32_t> o1(k);
+    for (size_t i = 0; i < nq; ++i) {
+        const auto t0 = clk::now();
+        symqg::QGBuilder::search_h2(g, &q[i * d], static_cast<uint32_t>(k), o1.data());
+        lat[i] = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+    }
+    std::sort(lat.begin(), lat.end());
+    const double p50 = lat[nq / 2], p99 = lat[std::min(nq - 1, (nq * 99) / 100)];
+
+    // all threads, one graph per thread
+    std::vector<std::unique_ptr<symqg::QuantizedGraph>> gs(threads);
+    for (int t = 0; t < threads; ++t) {
+        gs[t] = std::make_unique<symqg::QuantizedGraph>(n, deg, d);
+        symqg::QGBuilder::clone_rows(g, *gs[t]);
+        symqg::QGBuilder::load_signs(*gs[t], argv[6]);
+        gs[t]->set_ef(ef);
+    }
+    std::vector<double> qps;
+    for (size_t r = 0; r < rounds + 1; ++r) {
+        const auto t0 = clk::now();
+#pragma omp parallel num_threads(threads)
+        {
+            const int t = omp_get_thread_num();
+            std::vector<uint32_t> o(k);
+#pragma omp for schedule(dynamic, 16)
+            for (long i = 0; i < static_cast<long>(nq); ++i)
+                symqg::QGBuilder::search_h2(*gs[t], &q[i * d], static_cast<uint32_t>(k), o.data());
+        }
+        const double s = std::chrono::duration<double>(clk::now() - t0).count();
+        if (r > 0) qps.push_back(static_cast<double>(nq) / s);  // pass 0 is warm-up
+    }
+    std::sort(qps.begin(), qps.end());
+    if (std::strcmp(argv[13], "-") != 0) {
+        FILE* f = std::fopen(argv[13], "wb");
+        if (!f || std::fwrite(out.data(), 4, out.size(), f) != out.size()) return 2;
+        std::fclose(f);
+    }
+    if (std::strcmp(argv[14], "-") != 0) {
+        FILE* f = std::fopen(argv[14], "wb");
+        if (!f) return 2;
+        symqg::QGBuilder::dump_sidecar(g, f);
+        std::fclose(f);
+    }
+    std::printf(
+        "{\"p50_ms\": %.6f, \"p99_ms\": %.6f, \"qps_all\": %.3f, \"threads\": %d, "
+        "\"nq\": %zu, \"ef\": %zu, \"k\": %zu, \"sidecar_s\": %.3f}\n",
+        p50, p99, qps[qps.size() / 2], threads, nq, ef, k, side_s);
+    return 0;
+}
+
 // ops: triples (kind, id, dist bits); kind 0 = SearchBuffer.insert unless
 // is_full(dist), 1 = SearchBuffer.pop (if has_next), 2 = ResultBuffer.insert
 int cmd_buffers(int argc, char** argv) {
@@ -195,12 +295,13 @@ int cmd_buffers(int argc, char** argv) {
 
 int main(int argc, char** argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: ref_h2 search|buffers ...\n");
+        std::fprintf(stderr, "usage: ref_h2 search|time|buffers ...\n");
         return 1;
     }
     const std::string c = argv[1];
     int rc = 1;
     if (c == "search") rc = cmd_search(argc, argv);
+    else if (c == "time") rc = cmd_time(argc, argv);
     else if (c == "buffers") rc = cmd_buffers(argc, argv);
     if (rc == 1) std::fprintf(stderr, "bad arguments for %s\n", c.c_str());
     return rc;

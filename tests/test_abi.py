@@ -190,14 +190,24 @@ def test_fastscan_graph_and_serving_validation():
     ix.set_rotation("fht")
     assert L.lqg_h2_rotate(ix._h, _lib.ptr(q), 1, 0, _lib.ptr(q)) == 2  # no sidecar yet
     assert L.lqg_h2_rotate(ix._h, _lib.ptr(q), 1, 3, _lib.ptr(q)) == 1
-    # register caps: SQ8 takes 0, 6, 8 (FastScan's 5 is not an SQ8 setting)
-    for occ in (5, 7, -1):
+    # register caps: SQ8 takes -1 (uncapped), 0 (default), 6, 8 (FastScan's 5
+    # is not an SQ8 setting)
+    for occ in (5, 7, -2):
         with pytest.raises(_lib.LqgError) as e:
             ix.set_launch(0, waves_per_group=2, cache_bits=10, occupancy=occ)
         assert e.value.status == 1
-    ix.set_launch(0, waves_per_group=2, cache_bits=10, occupancy=6)
-    assert ix.get_launch(64)["occupancy"] == 6
+    for occ in (-1, 6):
+        ix.set_launch(0, waves_per_group=2, cache_bits=10, occupancy=occ)
+        assert ix.get_launch(64)["occupancy"] == occ
     ix.reset_launch(0)
+    # diagnostic knobs: the integer SQ8 kernel, the FastScan overflow table
+    assert L.lqg_set_sq8_integer(ix._h, 0) == 0
+    assert L.lqg_set_sq8_integer(ix._h, 1) == 0
+    assert L.lqg_set_sq8_integer(None, 1) == 1
+    for bits in (1, 7, 17):
+        assert L.lqg_h2_set_overflow_bits(ix._h, bits) == 1
+    for bits in (0, 8, 16, 14):
+        assert L.lqg_h2_set_overflow_bits(ix._h, bits) == 0
     # settings are per pool-capacity class; ef 129..192 is one class
     ix.set_launch(150, waves_per_group=3, cache_bits=7, occupancy=0)
     assert ix.get_launch(129)["cache_bits"] == 7 and ix.get_launch(192)["cache_bits"] == 7

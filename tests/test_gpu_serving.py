@@ -188,3 +188,111 @@ def test_requests_complete_before_their_batch(tmp_path, fastscan):
     assert np.array_equal(ids, want)
     assert st["completed"] == st["submitted"]
     assert np.median(lat[1:]) < 0.5 * lat[0], (np.median(lat[1:]), lat[0])
+
+
+def test_poll_capacity_below_k_raises_and_keeps_ticket(tmp_path):
+    """A poll whose result buffer is smaller than the request's k is refused
+    (no write past it) and the ticket stays valid for a correct poll."""
+    from liteqg_amd.serving import Server
+    from liteqg_amd.symphonyqg import LqgError
+
+    ix, arr = _index(tmp_path, n=2000)
+    q = np.random.default_rng(3).random(arr.dim, dtype=np.float32)
+    want = ix.search_batch(q[None], 12, ef=64, order="nearest")[0]
+    with Server(ix, lanes=1, max_batch=8, ef_max=64, kmax=16) as srv:
+        t = srv.submit(q, 64, 12)
+        assert t is not None
+        with pytest.raises(LqgError) as e:
+            srv.wait(t, 11)
+        assert e.value.status == 1
+        ids, _ = srv.wait(t, 12)
+        assert np.array_equal(ids, want)
+        with pytest.raises(LqgError):  # released now
+            srv.poll(t, 12)
+
+
+def test_overflow_fails_only_its_own_request(tmp_path):
+    """FastScan visited-set overflow is per request.  With no device overflow
+    table and a 64-entry LDS table, a query fails once its visited set passes
+    32 nodes; the device API marks exactly those queries (E = 0xFFFFFFFF, ids
+    0xFFFFFFFF), the host batch call fails naming the overflow, and in serving
+    only those requests fail while every other request of the same batches
+    is handed back with the results it has searched alone.  The kernel exits
+    normally (the limit is a host setting)."""
+    import ctypes as C
+
+    import torch
+
+    from liteqg_amd import _lib
+    from liteqg_amd.serving import Server
+    from liteqg_amd.symphonyqg import LqgError
+
+    ix, arr = _index(tmp_path, n=3000, fastscan=True)
+    rng = np.random.default_rng(11)
+    nq, k = 64, 10
+    Q = rng.random((nq, arr.dim), dtype=np.float32)
+    ef = np.where(np.arange(nq) % 4 == 0, 200, 4).astype(np.uint32)
+    want = ix.search_batch(Q, k, ef=ef)
+    ix.set_overflow_bits(0)
+    ix.set_launch(0, waves_per_group=1, cache_bits=6, occupancy=0)
+    try:
+        L = _lib.lib()
+        dq = torch.from_numpy(Q).cuda()
+        de = torch.from_numpy(ef.view(np.int32)).cuda()
+        dids = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+        dst = torch.empty((nq, 2), dtype=torch.int32, device="cuda")
+        _lib.check(L.lqg_search_batch_device(ix._h, C.c_void_p(dq.data_ptr()), nq,
+                                             C.c_void_p(de.data_ptr()), None, 200, k, 1,
+                                             C.c_void_p(dids.data_ptr()), None,
+                                             C.c_void_p(dst.data_ptr()), None))
+        torch.cuda.synchronize()
+        failed = dst.cpu().numpy().view(np.uint32)[:, 0] == 0xFFFFFFFF
+        got = dids.cpu().numpy().view(np.uint32)
+        assert failed.any() and (~failed).any()
+        assert (got[failed] == 0xFFFFFFFF).all()
+        assert np.array_equal(got[~failed], want[~failed])
+        with pytest.raises(LqgError, match="overflowed"):
+            ix.search_batch(Q, k, ef=ef)
+        with Server(ix, lanes=2, max_batch=16, ef_max=200, kmax=k) as srv:
+            tickets = [srv.submit(Q[i], int(ef[i]), k) for i in range(nq)]
+            for i, t in enumerate(tickets):
+                if failed[i]:
+                    with pytest.raises(LqgError, match="overflowed"):
+                        srv.wait(t, k)
+                else:
+                    ids, _ = srv.wait(t, k)
+                    assert np.array_equal(ids, want[i]), i
+    finally:
+        ix.set_overflow_bits(14)
+        ix.reset_launch(0)
+    assert np.array_equal(ix.search_batch(Q, k, ef=ef), want)
+
+
+def test_graph_refuses_replay_after_reload(tmp_path):
+    """A graph holds the index's device addresses; after the index is reloaded
+    (or its sidecar replaced) its replay is refused instead of reading freed
+    memory."""
+    import torch
+
+    from liteqg_amd.serving import BatchGraph
+    from liteqg_amd.symphonyqg import LqgError
+
+    ix, arr = _index(tmp_path, n=2000, fastscan=True)
+    Q = np.random.default_rng(5).random((100, arr.dim), dtype=np.float32)
+    dq = torch.from_numpy(Q).cuda()
+    ids = torch.empty((100, 10), dtype=torch.int32, device="cuda")
+    g = BatchGraph(ix, dq.data_ptr(), 100, ids.data_ptr(), 10, 50)
+    g.launch()
+    torch.cuda.synchronize()
+    ix.build_sidecar(seed=9)  # new records
+    with pytest.raises(LqgError, match="captured"):
+        g.launch()
+    g.close()
+    ix.set_mode("fastscan")
+    g = BatchGraph(ix, dq.data_ptr(), 100, ids.data_ptr(), 10, 50)
+    g.launch()
+    torch.cuda.synchronize()
+    ix.load(str(tmp_path / "s.index"))
+    with pytest.raises(LqgError, match="captured"):
+        g.launch()
+    g.close()
