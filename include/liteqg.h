@@ -140,6 +140,25 @@ lqg_status lqg_search_batch_device(lqg_index* index, const float* d_queries, uin
                                    uint32_t kmax, lqg_order order, uint32_t* d_results,
                                    float* d_dists, uint32_t* d_stats, void* stream);
 
+/* lqg_search_batch_device that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP-graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results.  The launch scratch (work-queue words, the
+ * FastScan overflow flag, GEMM-rotated queries) comes from one static device
+ * array of 1,052,928 bytes per GPU, held until the process ends, instead of
+ * buffers the library allocates.  Limits:
+ *   - the index must already be on the device (any earlier search on it);
+ *   - FastScan mode has no device overflow visited tables: a query whose
+ *     visited set passes half of its LDS table (256 to 2,048 expansions by pool
+ *     class) fails as in lqg_h2_set_overflow_bits(index, 0);
+ *   - with LQG_ROTATION_GEMM, nq x P must be at most 262,144 (LQG_ERR_ARG).
+ * Calls share the GPU's array: two calls that overlap on one GPU race. */
+lqg_status lqg_search_batch_device_noalloc(lqg_index* index, const float* d_queries,
+                                           uint64_t nq, const uint32_t* d_ef, const uint32_t* d_k,
+                                           uint32_t ef_max, uint32_t kmax, lqg_order order,
+                                           uint32_t* d_results, float* d_dists,
+                                           uint32_t* d_stats, void* stream);
+
 /* A batch launch captured into a HIP graph: lqg_search_batch_device with
  * fixed arguments (device arrays whose contents may change between replays),
  * recorded once and replayed with one hipGraphLaunch.  The graph holds the

@@ -63,6 +63,8 @@ _SIGS = {
     "lqg_search_batch": [_P, _P, C.c_uint64, _P, _P, C.c_uint32, C.c_int, _P, _P, _P],
     "lqg_search_batch_device": [_P, _P, C.c_uint64, _P, _P, C.c_uint32, C.c_uint32, C.c_int, _P,
                                 _P, _P, _P],
+    "lqg_search_batch_device_noalloc": [_P, _P, C.c_uint64, _P, _P, C.c_uint32, C.c_uint32,
+                                        C.c_int, _P, _P, _P, _P],
     "lqg_set_launch": [_P, C.c_uint32, _P],
     "lqg_get_launch": [_P, C.c_uint32, _P],
     "lqg_autotune": [_P, _P, C.c_uint64, C.c_uint32, C.c_uint32, _P],

@@ -58,6 +58,8 @@ lqg_status ensure_device(lqg_index* h) {
         return fail(LQG_ERR_DEVICE, std::string("device is ") + prop.gcnArchName +
                                         ", this library is built for gfx950 only");
     if (!h->stream) LQG_HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    lqg_status sst = static_scratch_setup();
+    if (sst != LQG_OK) return sst;
     if (!h->ev0) LQG_HIP_TRY(hipEventCreate(&h->ev0));
     if (!h->ev1) LQG_HIP_TRY(hipEventCreate(&h->ev1));
     const size_t nd = h->n * h->dim, ng = h->n * h->deg;

@@ -192,7 +192,7 @@ lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const
     // per-wave overflow visited tables in device memory (2^h2_gvis_bits
     // entries each), used once a query has expanded more nodes than half its
     // LDS table holds; none when h2_gvis_bits is 0
-    const uint32_t gbits = h->h2_gvis_bits;
+    const uint32_t gbits = sc->no_overflow ? 0u : h->h2_gvis_bits;
     if (gbits) LQG_HIP_TRY(sc->gvis.reserve((static_cast<size_t>(groups) * cfg.waves_per_group) << gbits));
     LQG_HIP_TRY(sc->err.reserve(1));
     LQG_HIP_TRY(sc->counter.reserve(1));

@@ -27,8 +27,12 @@ template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t cap = 0;  // elements
+    // A view of memory the buffer does not own (a region of a static device
+    // array): never allocated, grown or freed; a reserve beyond cap fails.
+    bool fixed = false;
     hipError_t reserve(size_t n) {
         if (n <= cap) return hipSuccess;
+        if (fixed) return hipErrorOutOfMemory;
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
@@ -37,6 +41,7 @@ struct DevBuf {
         return e;
     }
     void release() {
+        if (fixed) return;
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
@@ -53,6 +58,9 @@ struct Scratch {
     // Per-query completion flags the launch sets (host-coherent memory owned by
     // the caller, e.g. a serving lane), or null.
     uint32_t* done = nullptr;
+    // FastScan launches without the per-wave device overflow visited tables
+    // (a query then fails once its LDS table is half full).
+    bool no_overflow = false;
     void release() {
         counter.release();
         err.release();
@@ -171,6 +179,10 @@ namespace lqg {
 
 // Checks the device, creates the stream and uploads the host copy (once).
 lqg_status ensure_device(lqg_index* h);
+
+// Takes the current device's address of the static scratch array of the
+// allocation-free entry point (lqg_static.hip), once per device.
+lqg_status static_scratch_setup();
 
 // Enqueues one search launch over device arrays, in the handle's mode, with
 // scratch `sc` (null: the handle's own).
