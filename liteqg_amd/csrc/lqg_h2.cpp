@@ -173,10 +173,12 @@ void h2_reset(lqg_index* h) {
     h->mode = LQG_MODE_SQ8;
 }
 
-lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
-                        const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, uint32_t* d_results,
-                        float* d_dists, uint32_t* d_stats, hipStream_t s, Scratch* sc) {
-    if (!h->h2_ready) return fail(LQG_ERR_STATE, "FastScan mode needs a sidecar");
+namespace {
+
+// Launch geometry of a FastScan batch of nq queries (ef up to ef_max, k up to
+// kmax): the class's settings with waves per group and groups fitted.
+lqg_status h2_batch_cfg(const lqg_index* h, uint64_t nq, uint32_t ef_max, uint32_t kmax,
+                        H2Cfg* out) {
     const int cls = ef_class(ef_max);
     if (cls == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported on the device");
     H2Cfg cfg = h2_effective_launch(h, cls);
@@ -189,15 +191,49 @@ lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const
     const int groups = h2_groups(ix, cfg, kmax, static_cast<uint32_t>(nq));
     if (groups <= 0) return fail(LQG_ERR_DEVICE, "FastScan: no launch geometry");
     cfg.groups = groups;
-    // per-wave overflow visited tables in device memory (2^h2_gvis_bits
-    // entries each), used once a query has expanded more nodes than half its
-    // LDS table holds; none when h2_gvis_bits is 0
+    *out = cfg;
+    return LQG_OK;
+}
+
+// Device scratch of such a batch: per-wave overflow visited tables in device
+// memory (2^h2_gvis_bits entries each, none when that is 0 or the scratch
+// has no_overflow set), used once a query has expanded more nodes than half
+// its LDS table holds; the overflow flag and the work-queue head; the
+// GEMM-rotated queries.
+lqg_status h2_reserve(const lqg_index* h, const H2Cfg& cfg, uint64_t nq, Scratch* sc) {
     const uint32_t gbits = sc->no_overflow ? 0u : h->h2_gvis_bits;
-    if (gbits) LQG_HIP_TRY(sc->gvis.reserve((static_cast<size_t>(groups) * cfg.waves_per_group) << gbits));
+    if (gbits)
+        LQG_HIP_TRY(sc->gvis.reserve((static_cast<size_t>(cfg.groups) * cfg.waves_per_group)
+                                     << gbits));
     LQG_HIP_TRY(sc->err.reserve(1));
     LQG_HIP_TRY(sc->counter.reserve(1));
+    if (h->h2_rotation == LQG_ROTATION_GEMM) LQG_HIP_TRY(sc->rot.reserve(nq * h->h2_p));
+    return LQG_OK;
+}
+
+}  // namespace
+
+lqg_status h2_prepare_scratch(lqg_index* h, uint64_t nq, uint32_t ef_max, uint32_t kmax,
+                              Scratch* sc) {
+    if (!h->h2_ready) return LQG_OK;
+    H2Cfg cfg;
+    lqg_status st = h2_batch_cfg(h, nq, ef_max, kmax, &cfg);
+    if (st != LQG_OK) return st;
+    return h2_reserve(h, cfg, nq, sc);
+}
+
+lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const uint32_t* d_ef,
+                        const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, uint32_t* d_results,
+                        float* d_dists, uint32_t* d_stats, hipStream_t s, Scratch* sc) {
+    if (!h->h2_ready) return fail(LQG_ERR_STATE, "FastScan mode needs a sidecar");
+    H2Cfg cfg;
+    lqg_status st = h2_batch_cfg(h, nq, ef_max, kmax, &cfg);
+    if (st != LQG_OK) return st;
+    st = h2_reserve(h, cfg, nq, sc);
+    if (st != LQG_OK) return st;
+    const H2Dev ix = h->h2dev();
+    const uint32_t gbits = sc->no_overflow ? 0u : h->h2_gvis_bits;
     if (h->h2_rotation == LQG_ROTATION_GEMM) {
-        LQG_HIP_TRY(sc->rot.reserve(nq * h->h2_p));
         LQG_HIP_TRY(h2_launch_rotate_queries(ix, d_queries, static_cast<uint32_t>(nq), 1,
                                              sc->rot.p, s));
     }

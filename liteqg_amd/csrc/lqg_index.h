@@ -196,6 +196,12 @@ lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const
                         const uint32_t* d_k, uint32_t ef_max, uint32_t kmax, uint32_t* d_results,
                         float* d_dists, uint32_t* d_stats, hipStream_t s, Scratch* sc);
 
+// Reserves the device scratch a FastScan batch of up to nq queries (ef up to
+// ef_max, k up to kmax) needs, so that later launches allocate nothing (no-op
+// without a sidecar).
+lqg_status h2_prepare_scratch(lqg_index* h, uint64_t nq, uint32_t ef_max, uint32_t kmax,
+                              Scratch* sc);
+
 // Launch settings of the FastScan kernel used when none were set for a class.
 H2Cfg h2_default_launch(int ef_cap);
 

@@ -413,10 +413,21 @@ lqg_status lqg_serve_start(lqg_index* h, const lqg_serve_config* cfg, lqg_server
             e = hipHostGetDevicePointer(reinterpret_cast<void**>(&l.g_dist), l.h_dist, 0);
         if (e == hipSuccess)
             e = hipHostGetDevicePointer(reinterpret_cast<void**>(&l.g_flag), l.h_flag, 0);
+        // every lane's launch scratch is allocated here, for its largest
+        // batch: a hipMalloc / hipFree in the dispatcher can wait for the
+        // device, and with it hold back every lane's hand-back
+        if (e == hipSuccess) e = l.scratch.counter.reserve(2 * lqg::kQueueWords + 1);
     }
     if (e != hipSuccess) {
         destroy(s);
         return lqg::hip_fail(e, "serving buffers");
+    }
+    for (Lane& l : s->lanes) {
+        st = lqg::h2_prepare_scratch(h, cfg->max_batch, cfg->ef_max, cfg->kmax, &l.scratch);
+        if (st != LQG_OK) {
+            destroy(s);
+            return st;
+        }
     }
     try {
         s->worker = std::thread(dispatcher, s);
