@@ -1,0 +1,819 @@
+// h2_kernel.h -- device code of the reference's FastScan RaBitQ query
+// path, QuantizedGraph::search_qg (qg.hpp:252-287): query preparation
+// (QGQuery::query_prepare, qg_query.hpp:28-43), FastScan estimates of every
+// neighbour of an expanded node (QGScanner::scan_neighbors,
+// qg_scanner.hpp:99-155), the exact L2 distance of the node itself
+// (space/l2.hpp:16-34) kept in a top-k ResultBuffer, update_results
+// (qg.hpp:395-418), and the sidecar build that fills the codes and factors
+// the scan reads (quantization/rabitq.hpp:23-108).
+//
+// One wavefront walks one query.
+//
+// Query preparation runs in LDS: the FHT rotation (sign multiply, then the
+// unnormalised Walsh-Hadamard butterflies with strides 1, 2, ..., P/2, the
+// operation order of utils/rotator.hpp:88-114 and FFHT), the [lo, hi] range
+// seeded with 0, 6-bit quantisation and the 16-entry uint8 LUT of each
+// 4-dimension codebook.  The two constant divisions are evaluated as the
+// reference's -Ofast build folds them: width = (hi - lo) * fl(1/63) and
+// 1/width = fl(1/fl(1/63)) / (hi - lo); q = lround(double(f) + 0.5).
+//
+// SearchBuffer (utils/buffer.hpp:11-78) semantics: entries sorted by
+// estimate, a new entry placed before equal ones, the last dropped when full,
+// an entry skipped when the buffer is full and its estimate is above the
+// last, duplicates of an id allowed, pop = first unchecked entry.  Inserting
+// the neighbours of one node one by one under these rules leaves the ef
+// smallest entries of (buffer + all inserted) in the order
+//   (estimate ascending, later insertion first)
+// so the kernel keeps the buffer as a register pool sorted by the 64-bit key
+//   sortable(estimate) << 32 | (2^31 - 1 - seq) << 1 | checked
+// (seq = insertion number) and merges each node's neighbours in one pass, as
+// the SQ8 kernel does.  The visited set (nodes expanded, plus the nodes
+// update_results adds) is exact: an LDS open-addressing table, moved to a
+// per-wave table in device memory if it passes half its capacity.
+//
+// FastScan: a lane holds the codes of one neighbour (deg >= 64) or half of
+// them (deg = 32) and adds LUT bytes looked up in LDS; sums are taken mod
+// 2^16 and read back as int16, as the AVX-512 code does (qg_scanner.hpp:
+// 117-135); est = (f * fac_dq) * width + fma(fac_vq, lo, triple_x + sqr_y), the
+// operation order of the reference build (qg_scanner.hpp:23-48 as compiled).
+#pragma once
+
+#include <math.h>
+
+#include "lqg_internal.h"
+#include "wave.h"
+
+namespace lqg {
+
+struct Layout {
+    uint32_t qv, tb, qb, lut, st, hv, wave_bytes;
+    int has_qv;
+};
+
+__host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
+
+// Per-wave LDS layout.  with_qv: keep the query in LDS (the probe kernel, and
+// searches with dim > 16 * kRegJ, whose exact distances read dims past the
+// registers' from there).
+__host__ __device__ inline Layout h2_layout(uint32_t dim, uint32_t p, uint32_t ef_cap,
+                                            uint32_t hash_bits, bool with_qv) {
+    Layout l;
+    uint32_t o = 0;
+    l.has_qv = with_qv ? 1 : 0;
+    l.qv = o;   o += with_qv ? align16(4 * dim) : 0u;
+    l.lut = o;  o += align16(4 * p + 16);
+    // one region for the preparation buffers (tb, qb) and, once the LUT is
+    // built, the pool merge staging (ef_cap records and a spare)
+    const uint32_t prep = align16(4 * p) + align16(p);
+    const uint32_t stage = 16u * (ef_cap + 1);
+    l.tb = o;
+    l.qb = o + align16(4 * p);
+    l.st = o;
+    o += prep > stage ? prep : stage;
+    l.hv = o;   o += 4u << hash_bits;
+    l.wave_bytes = o;
+    return l;
+}
+
+namespace {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr float kInv63 = 1.0f / 63.0f;  // width = (hi - lo) * fl(1/63)
+constexpr float kC63 = 1.0f / kInv63;   // 1/width = fl(1/fl(1/63)) / (hi - lo)
+
+// Estimate bits that sort like the floats (-0 taken as +0, as a float
+// comparison does).
+__device__ __forceinline__ uint32_t sortable(float x) {
+    const uint32_t u = __float_as_uint(x == 0.0f ? 0.0f : x);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ uint64_t sb_key(float est, uint32_t seq) {
+    return (static_cast<uint64_t>(sortable(est)) << 32) |
+           (static_cast<uint64_t>(0x7FFFFFFFu - seq) << 1);
+}
+
+struct H2Lds {
+    float* qv;        // query (dim), only where dim > 16 * kRegJ (the rest sits in registers)
+    float* tb;        // rotated query during preparation; q - x in exact_l2 (p floats)
+    uint8_t* qb;      // quantised query (p bytes)
+    uint8_t* lut;     // 4p bytes (+16 bytes of bank skew for 2 lanes per vector)
+    uint4* st;        // pool merge staging, once the LUT is built (shares tb's memory):
+                      // EF_CAP + 1 records {key lo, key hi, id, 0}
+    uint32_t* hv;     // visited table (1 << hash_bits)
+};
+
+__device__ __forceinline__ H2Lds carve(unsigned char* base, const Layout& l) {
+    H2Lds w;
+    w.qv = l.has_qv ? reinterpret_cast<float*>(base + l.qv) : nullptr;
+    w.tb = reinterpret_cast<float*>(base + l.tb);
+    w.qb = base + l.qb;
+    w.lut = base + l.lut;
+    w.st = reinterpret_cast<uint4*>(base + l.st);
+    w.hv = reinterpret_cast<uint32_t*>(base + l.hv);
+    return w;
+}
+
+// ---------------------------------------------------------------------------
+// Query preparation (qg_query.hpp:28-43) into LDS: qv (raw query), lut.
+struct Prep {
+    float lo, width;
+    int32_t sumq;
+};
+
+// In-place unnormalised Walsh-Hadamard transform of t[0..p) in LDS by one
+// wave, butterflies (a, b) -> (a + b, a - b) with strides 1, 2, ..., p/2.
+__device__ void fht_lds(float* t, uint32_t p, uint32_t lane) {
+    for (uint32_t lh = 0; (1u << lh) < p; ++lh) {
+        const uint32_t h = 1u << lh;
+        for (uint32_t b = lane; b < p / 2; b += kWave) {
+            const uint32_t j = ((b >> lh) << (lh + 1)) | (b & (h - 1u));
+            const float a = t[j], c = t[j + h];
+            t[j] = a + c;
+            t[j + h] = a - c;
+        }
+        wave_sync();
+    }
+}
+
+// rsrc: the query already rotated (p floats, the GEMM rotation), or null to
+// rotate here with the FHT.
+__device__ Prep prepare(const H2Dev& ix, const float* qsrc, const float* rsrc, const H2Lds& w,
+                        uint32_t lpv, uint32_t lane) {
+    const uint32_t dim = ix.dim, p = ix.p;
+    if (w.qv)
+        for (uint32_t i = lane; i < dim; i += kWave) w.qv[i] = qsrc[i];
+    if (rsrc) {
+        for (uint32_t i = lane; i < p; i += kWave) w.tb[i] = rsrc[i];
+        wave_sync();
+    } else {
+        for (uint32_t i = lane; i < p; i += kWave) w.tb[i] = i < dim ? qsrc[i] * ix.signs[i] : 0.0f;
+        wave_sync();
+        fht_lds(w.tb, p, lane);
+    }
+    // data_range (utils/scalar_quantize.hpp:11-38): min and max seeded with 0
+    float lo = 0.0f, hi = 0.0f;
+    for (uint32_t i = lane; i < p; i += kWave) {
+        const float t = w.tb[i];
+        lo = t < lo ? t : lo;
+        hi = t > hi ? t : hi;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const float a = __shfl_xor(lo, o), b = __shfl_xor(hi, o);
+        lo = a < lo ? a : lo;
+        hi = b > hi ? b : hi;
+    }
+    const float width = (hi - lo) * kInv63;
+    const float ow = kC63 / (hi - lo);
+    // quantize (scalar_quantize.hpp:40-58): q = lround(double(f) + 0.5); for
+    // f >= 0 that is floor(f + 1) (exact in double).  A NaN or infinite f
+    // (hi == lo) gives 0, the low byte of the x86 conversion's result.
+    int32_t sq = 0;
+    for (uint32_t i = lane; i < p; i += kWave) {
+        const float f = (w.tb[i] - lo) * ow;
+        uint32_t q = 0;
+        if (f <= 1.0e9f) q = static_cast<uint32_t>(floor(static_cast<double>(f) + 1.0)) & 0xFFu;
+        w.qb[i] = static_cast<uint8_t>(q);
+        sq += static_cast<int32_t>(q);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
+    wave_sync();
+    // pack_lut_impl (fastscan_impl.hpp:258-270): entry j of codebook c is the
+    // uint8 sum of q[4c + 3 - b] over the set bits b of j.  With two lanes per
+    // neighbour the second half of the codebooks sits 16 bytes further, so the
+    // two halves of a wavefront read different LDS banks.
+    const uint32_t ncb = p / 4, cpl = ncb / lpv;
+    for (uint32_t idx = lane; idx < 16u * ncb; idx += kWave) {
+        const uint32_t c = idx >> 4, j = idx & 15u;
+        uint32_t s = 0;
+#pragma unroll
+        for (uint32_t b = 0; b < 4; ++b)
+            if ((j >> b) & 1u) s += w.qb[4 * c + 3 - b];
+        w.lut[16 * c + (c >= cpl ? 16u : 0u) + j] = static_cast<uint8_t>(s & 0xFFu);
+    }
+    wave_sync();
+    Prep r;
+    r.lo = lo;
+    r.width = width;
+    r.sumq = sq;
+    return r;
+}
+
+// The same distance with the partials in registers: lane p < 16 runs the FMA
+// chain over dims p, p + 16, ... (the first kRegJ of them from xr / qr, which
+// the caller loaded with load_partials, the rest from the row and w.qv), the
+// tree is taken with DPP row shifts (lane p adds lane p + 8, p + 4, p + 2,
+// p + 1 of its row: the pairs of reduce_l2), and lane 0's sum is broadcast.
+// Lanes 16..63 repeat the work of lane (lane & 15), so no load or step needs
+// a lane mask; only lanes 0..15 feed the tree.  Registers past the chain
+// length hold 0, and fma(0, 0, s) = s, so their steps change nothing.
+constexpr int kRegJ = 8;
+
+__device__ __forceinline__ void load_partials(float (&r)[kRegJ], const float* src, uint32_t dim,
+                                              uint32_t lane) {
+    const uint32_t nj = dim / 16, p = lane & 15u;
+#pragma unroll
+    for (int u = 0; u < kRegJ; ++u) {
+        r[u] = 0.0f;
+        if (static_cast<uint32_t>(u) < nj) r[u] = src[p + 16u * static_cast<uint32_t>(u)];
+    }
+}
+
+__device__ __forceinline__ float dpp_shl(float x, int ctrl_n) {
+    // row_shl:n, lanes past the row end read 0
+    int r = 0;
+    switch (ctrl_n) {
+        case 8: r = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x108, 0xF, 0xF, false); break;
+        case 4: r = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x104, 0xF, 0xF, false); break;
+        case 2: r = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x102, 0xF, 0xF, false); break;
+        default: r = __builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x101, 0xF, 0xF, false); break;
+    }
+    return __int_as_float(r);
+}
+
+__device__ __forceinline__ float exact_l2_regs(const float (&xr)[kRegJ], const float (&qr)[kRegJ],
+                                               const float* row, const H2Lds& w, uint32_t dim,
+                                               uint32_t lane) {
+    const uint32_t nj = dim / 16;  // chain length of every partial
+    const uint32_t p = lane & 15u;
+    float s = 0.0f;
+#pragma unroll
+    for (int u = 0; u < kRegJ; ++u) {
+        const float t = qr[u] - xr[u];
+        s = fmaf(t, t, s);
+    }
+    for (uint32_t j = kRegJ; j < nj; ++j) {
+        const float t = w.qv[p + 16 * j] - row[p + 16 * j];
+        s = fmaf(t, t, s);
+    }
+    s = s + dpp_shl(s, 8);
+    s = s + dpp_shl(s, 4);
+    s = s + dpp_shl(s, 2);
+    s = s + dpp_shl(s, 1);
+    float r = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(s)));
+    for (uint32_t i = nj * 16; i < dim; ++i) {
+        const float t = w.qv[i] - row[i];
+        r = fmaf(t, t, r);
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Exact visited set (nodes expanded, and the nodes update_results adds):
+// open addressing over 2^bits slots (empty = 0xFFFFFFFF), in LDS or, for a
+// query whose set would pass half of its LDS table, in a per-wave table in
+// device memory (the query is then searched again from the start with that
+// table: GLOBAL = true).  An id's probe sequence starts at the first slot of
+// its 4-slot bucket and runs through consecutive slots.  Nothing is ever
+// removed, so the filled slots of a bucket are a prefix of it: a lane tests a
+// whole bucket with one 16-byte LDS read and stops at a bucket whose last
+// slot is empty, and a uniform probe reads 64 consecutive slots in one
+// instruction, one per lane.  The device table is accessed with atomics, so
+// every lane sees every insertion.
+template <bool GLOBAL>
+struct Visited {
+    uint32_t* tab;  // 2^bits slots (LDS, or this wave's device table)
+    uint32_t bits, count;
+};
+
+__device__ __forceinline__ uint32_t vbucket(uint32_t id, uint32_t bits) {
+    return ((id * 2654435761u) >> (32u - bits)) & ~3u;
+}
+
+__device__ __forceinline__ uint32_t g_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+enum : uint32_t { kVisNew = 0, kVisSeen = 1, kVisFull = 2 };
+
+// For an id equal on every lane: kVisSeen if it is in the set; otherwise
+// inserts it (kVisNew), or, if the set would pass half of its table, inserts
+// nothing (kVisFull).
+template <bool GLOBAL>
+__device__ uint32_t v_test_set(Visited<GLOBAL>& v, uint32_t id, uint32_t lane) {
+    const uint32_t mask = (1u << v.bits) - 1u;
+    uint32_t h = vbucket(id, v.bits);
+    uint32_t fe = 0, slot = 0;
+    for (;;) {
+        slot = (h + lane) & mask;
+        const uint32_t e = GLOBAL ? g_load(&v.tab[slot]) : v.tab[slot];
+        const uint64_t hit = __ballot(e == id), emp = __ballot(e == kEmpty);
+        if (hit | emp) {
+            const uint32_t fh = hit ? static_cast<uint32_t>(__builtin_ctzll(hit)) : 64u;
+            fe = emp ? static_cast<uint32_t>(__builtin_ctzll(emp)) : 64u;
+            if (fh < fe) return kVisSeen;
+            break;
+        }
+        h = (h + kWave) & mask;
+    }
+    if (v.count + 1u > (1u << v.bits) / 2u) return kVisFull;
+    if (GLOBAL) {
+        // the first empty slot may have been taken meanwhile by no one else
+        // (the table is this wave's), so the CAS lands there
+        if (lane == fe) atomicCAS(&v.tab[slot], kEmpty, id);
+        __threadfence();
+    } else {
+        if (lane == fe) v.tab[slot] = id;
+    }
+    wave_sync();
+    ++v.count;
+    return kVisNew;
+}
+
+// For each lane with `active` set: whether its id is in the set (false on
+// the other lanes).  Every lane reads each round (inactive lanes an in-range
+// bucket), so the loop has no divergent branch.
+template <bool GLOBAL>
+__device__ __forceinline__ bool v_contains_lanes(const Visited<GLOBAL>& v, uint32_t id,
+                                                 bool active) {
+    bool found = false;
+    const uint32_t bmask = (1u << (v.bits - 2u)) - 1u;
+    uint32_t b = vbucket(id, v.bits) >> 2;
+    for (;;) {
+        uint4 e;
+        if (GLOBAL) {
+            const uint32_t* t = v.tab + 4u * b;
+            e = make_uint4(g_load(t), g_load(t + 1), g_load(t + 2), g_load(t + 3));
+        } else {
+            e = reinterpret_cast<const uint4*>(v.tab)[b];
+        }
+        const bool f = active && (e.x == id || e.y == id || e.z == id || e.w == id);
+        found = found || f;
+        active = active && !f && e.w != kEmpty;
+        if (__ballot(active) == 0) break;
+        b = (b + 1u) & bmask;
+    }
+    return found;
+}
+
+// ---------------------------------------------------------------------------
+// ResultBuffer (utils/buffer.hpp:81-120) in registers: entry j is slot j / 64
+// of lane j % 64 of rd / ri, KS slots (k <= 64 * KS), sorted by exact
+// distance, empty entries +inf.  insert (buffer.hpp:84-96): placed before
+// equal distances, skipped when full and above the last; the entry pushed
+// past k is dropped.
+template <int KS>
+struct ResBuf {
+    float rd[KS];
+    uint32_t ri[KS];
+    uint32_t size;
+    float tail;  // distance of entry k - 1 once full
+};
+
+template <int KS>
+__device__ __forceinline__ void rb_init(ResBuf<KS>& r) {
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        r.rd[s] = INFINITY;
+        r.ri[s] = 0u;
+    }
+    r.size = 0;
+    r.tail = INFINITY;
+}
+
+__device__ __forceinline__ float wave_shr1(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t x) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x138, 0xF, 0xF, false));
+}
+
+template <int KS>
+__device__ __forceinline__ void rb_insert(ResBuf<KS>& r, uint32_t kq, uint32_t id, float x,
+                                          uint32_t lane) {
+    if (kq == 0 || (r.size == kq && x > r.tail)) return;
+    // entries below x among the first k (entries past k are stale copies)
+    uint32_t pos = 0;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const uint32_t j = static_cast<uint32_t>(s * kWave) + lane;
+        pos += static_cast<uint32_t>(__popcll(__ballot(j < kq && r.rd[s] < x)));
+    }
+    // entries at or after pos move up one place (lane 63 of a slot into lane
+    // 0 of the next), then x goes to pos
+#pragma unroll
+    for (int s = KS - 1; s >= 0; --s) {
+        const uint32_t j = static_cast<uint32_t>(s * kWave) + lane;
+        float sd = wave_shr1(r.rd[s]);
+        uint32_t si = wave_shr1(r.ri[s]);
+        if (s > 0) {
+            const float cd = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(r.rd[s - 1]), 63));
+            const uint32_t ci = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(r.ri[s - 1]), 63));
+            sd = lane == 0 ? cd : sd;
+            si = lane == 0 ? ci : si;
+        }
+        r.rd[s] = j > pos ? sd : (j == pos ? x : r.rd[s]);
+        r.ri[s] = j > pos ? si : (j == pos ? id : r.ri[s]);
+    }
+    r.size = min(r.size + 1u, kq);
+    if (r.size == kq) {
+        const uint32_t t = kq - 1u;
+        float tl = r.rd[0];
+#pragma unroll
+        for (int s = 0; s < KS; ++s)
+            if (static_cast<uint32_t>(s) == t / kWave) tl = r.rd[s];
+        r.tail = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tl), static_cast<int>(t % kWave)));
+    }
+}
+
+// Id of entry t (t < size; uniform t).
+template <int KS>
+__device__ __forceinline__ uint32_t rb_id(const uint32_t (&ri)[KS], uint32_t t) {
+    uint32_t v = ri[0];
+#pragma unroll
+    for (int s = 0; s < KS; ++s)
+        if (static_cast<uint32_t>(s) == t / kWave) v = ri[s];
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(t % kWave)));
+}
+
+// ---------------------------------------------------------------------------
+// FastScan sum of one lane: CW code words (8 codebooks each) starting at
+// codebook c0, LUT base already offset for the lane's half.
+template <int CW>
+__device__ __forceinline__ uint32_t scan_sum(const uint32_t (&cw)[CW], const uint8_t* lut) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < CW; ++k) {
+        const uint32_t word = cw[k];
+#pragma unroll
+        for (int n = 0; n < 8; ++n) s += lut[16 * (8 * k + n) + ((word >> (4 * n)) & 15u)];
+    }
+    return s;
+}
+
+template <int CW>
+__device__ __forceinline__ void load_codes(uint32_t (&cw)[CW], const uint8_t* src) {
+    if constexpr (CW == 1) {
+        cw[0] = *reinterpret_cast<const uint32_t*>(src);
+    } else if constexpr (CW == 2) {
+        const uint2 v = *reinterpret_cast<const uint2*>(src);
+        cw[0] = v.x;
+        cw[1] = v.y;
+    } else {
+#pragma unroll
+        for (int k = 0; k < CW; k += 4) {
+            const uint4 v = *reinterpret_cast<const uint4*>(src + 4 * k);
+            cw[k] = v.x;
+            cw[k + 1] = v.y;
+            cw[k + 2] = v.z;
+            cw[k + 3] = v.w;
+        }
+    }
+}
+
+// Estimates of the neighbours a lane holds for one node (pass of VP
+// neighbours starting at vb).  Returns the estimate on every lane of the
+// neighbour's lanes.
+template <int CW>
+__device__ __forceinline__ float lane_estimate(const uint32_t (&cw)[CW], const uint8_t* lut_part,
+                                               uint32_t lpv, const Prep& pr, float sqr_y,
+                                               float tx, float dq, float vq) {
+    uint32_t s = scan_sum<CW>(cw, lut_part);
+    // the two lanes of a neighbour: DPP quad_perm [1, 0, 3, 2]
+    if (lpv == 2)
+        s += static_cast<uint32_t>(
+            __builtin_amdgcn_update_dpp(0, static_cast<int>(s), 0xB1, 0xF, 0xF, false));
+    const int32_t r = static_cast<int32_t>(static_cast<int16_t>(static_cast<uint16_t>(s & 0xFFFFu)));
+    const float f = static_cast<float>(r * 2 - pr.sumq);
+    const float b = fmaf(vq, pr.lo, tx + sqr_y);
+    const float c = (f * dq) * pr.width;
+    return c + b;
+}
+
+// ---------------------------------------------------------------------------
+// One query.  GLOBAL = false (the search kernel): the visited set in the
+// wave's LDS table; if it would pass half of it, the query is appended to
+// the launch's overflow list and false returned with nothing written (the
+// overflow kernel searches it again from the start), or, where the launch
+// has no overflow list, the query fails.  GLOBAL = true (the overflow
+// kernel): the set in the wave's device table gtab.  A failed query (past
+// half of its last table, or no device table) has E = 0xFFFFFFFF in the
+// stats and ids 0xFFFFFFFF, sets job.err and its kFlagFailed flag.
+// Appends query q to the launch's overflow list (ovf[0] = count, ovf[2 + i]
+// = entries); returns false.
+__device__ __forceinline__ bool h2_defer(const H2Job& job, uint32_t q, uint32_t lane) {
+    if (lane == 0) job.ovf[2 + atomicAdd(&job.ovf[0], 1u)] = q;
+    wave_sync();
+    return false;
+}
+
+template <int EF_CAP, int CW, int KS, bool GLOBAL>
+__device__ bool h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, const H2Lds& w,
+                              uint32_t hash_bits, uint32_t* gtab, uint32_t lane) {
+    constexpr int S = EF_CAP / kWave;
+    const uint32_t ef =
+        uniform(max(1u, min(job.ef ? job.ef[q] : job.ef_all, static_cast<uint32_t>(EF_CAP))));
+    const uint32_t kq = uniform(min(job.k ? job.k[q] : job.k_all, job.kstride));
+    const uint32_t deg = ix.deg, dim = ix.dim;
+    const uint32_t lpv = deg >= 64 ? 1u : 2u;
+    const uint32_t vp = kWave / lpv;              // neighbours per pass
+    const uint32_t cpl = (ix.p / 4) / lpv;        // codebooks per lane
+    const uint32_t code_bytes = ix.p / 8;         // per neighbour
+    const uint32_t part = lane % lpv;
+    const uint8_t* lut_part = w.lut + 16u * cpl * part + (part ? 16u : 0u);
+
+    const Prep pr = prepare(ix, job.queries + static_cast<size_t>(q) * dim,
+                            job.rot ? job.rot + static_cast<size_t>(q) * ix.p : nullptr, w, lpv,
+                            lane);
+
+    Visited<GLOBAL> vis;
+    vis.tab = GLOBAL ? gtab : w.hv;
+    vis.bits = GLOBAL ? job.gvis_bits : hash_bits;
+    vis.count = 0;
+    bool failed = GLOBAL && gtab == nullptr;
+    if (!failed) {
+        for (uint32_t i = lane; i < (1u << vis.bits); i += kWave) vis.tab[i] = kEmpty;
+        if (GLOBAL) __threadfence();
+    }
+    // the merge staging reads back every position below EF_CAP; positions the
+    // pool has not reached yet must read as empty entries
+    for (uint32_t i = lane; i <= static_cast<uint32_t>(EF_CAP); i += kWave)
+        w.st[i] = make_uint4(kEmpty, kEmpty, kEmpty, 0u);
+    wave_sync();
+
+    // search_pool_.insert(entry_point_, FLT_MAX) (qg.hpp:260); empty entries
+    // are ~0, whose bit 0 (checked) keeps them from being popped
+    uint64_t P[S];
+    uint32_t I[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        P[s] = ~0ull;
+        I[s] = kEmpty;
+    }
+    if (lane == 0) {
+        P[0] = sb_key(3.402823466e+38f, 0);
+        I[0] = ix.ep;
+    }
+    uint32_t size = 1, seq = 1, n_exp = 0, n_pop = 0;
+    ResBuf<KS> rb;
+    rb_init(rb);
+    // query dims (lane & 15) + 16u (u < kRegJ) for the partials of exact_l2_regs
+    float qr[kRegJ];
+    load_partials(qr, job.queries + static_cast<size_t>(q) * dim, dim, lane);
+    // per-lane offsets inside a node record for the neighbour this lane scans
+    // in a pass (the pass's first neighbour added per pass)
+    const uint32_t lv = lane / lpv;
+    const uint32_t code_off = ix.off_codes + lv * code_bytes + part * (code_bytes / lpv);
+
+    while (!failed) {
+        // pop: first unchecked entry (buffer.hpp:52-60)
+        uint64_t pm = 0;
+        int fs = S;
+#pragma unroll
+        for (int s = S - 1; s >= 0; --s) {
+            const uint64_t m = __ballot((static_cast<uint32_t>(P[s]) & 1u) == 0u);
+            if (m) {
+                pm = m;
+                fs = s;
+            }
+        }
+        if (fs == S) break;
+        const int pl = static_cast<int>(__builtin_ctzll(pm));
+        uint32_t cur = 0;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (s == fs) {
+                cur = readlane32(I[s], pl);
+                P[s] |= static_cast<int>(lane) == pl ? 1ull : 0ull;
+            }
+        }
+        ++n_pop;
+        const uint32_t vs = v_test_set(vis, cur, lane);
+        if (vs == kVisSeen) continue;
+        if (vs == kVisFull) {
+            if (!GLOBAL && job.ovf) return h2_defer(job, q, lane);
+            failed = true;
+            break;
+        }
+        ++n_exp;
+
+        const unsigned char* rec = ix.rec + static_cast<size_t>(cur) * ix.stride;
+        const float* raw = reinterpret_cast<const float*>(rec);
+        const float* fac = reinterpret_cast<const float*>(rec + ix.off_fac);
+        const uint32_t* ids = reinterpret_cast<const uint32_t*>(rec + ix.off_ids);
+        // the node's raw row for the exact distance: dims (lane & 15) + 16u
+        float xr[kRegJ];
+        load_partials(xr, raw, dim, lane);
+        float sqr_y = 0.0f;
+
+        for (uint32_t vb = 0; vb < deg; vb += vp) {
+            // the pass's loads first (codes, factors, id); lanes past the last
+            // neighbour (a pass of a degree that is not a multiple of the
+            // pass width) read the last one and take no part
+            const uint32_t vi = vb + lv;
+            const bool act = vi < deg;
+            const uint32_t vr = act ? vi : deg - 1u;
+            uint32_t cw[CW];
+            load_codes<CW>(cw, rec + code_off + (vr - lv) * code_bytes);
+            const uint32_t nid = ids[vr];
+            const float tx = fac[vr];
+            const float dq = fac[deg + vr];
+            const float vq = fac[2 * deg + vr];
+            // scan_neighbors (qg.hpp:355): exact distance to the node
+            if (vb == 0) sqr_y = exact_l2_regs(xr, qr, raw, w, dim, lane);
+            const float est = lane_estimate<CW>(cw, lut_part, lpv, pr, sqr_y, tx, dq, vq);
+            // is_full / visited_.get (qg.hpp:375-377); candidate i gets insertion
+            // number seq + i.  The visited table is probed only by the lanes the
+            // pool would take.
+            const uint64_t key = sb_key(est, seq + vi);
+            bool cand = act && part == 0;
+            if (size == ef) {
+                const uint32_t t = size - 1u;
+                uint64_t lastv = P[0];
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+                    if (static_cast<uint32_t>(s) == t / kWave) lastv = P[s];
+                const uint64_t last = readlane64(lastv, static_cast<int>(t % kWave));
+                cand = cand && key < last;
+            }
+            if (__ballot(cand) == 0) continue;
+            cand = cand && !v_contains_lanes(vis, nid, cand);
+            uint64_t mm = __ballot(cand);
+            if (mm == 0) continue;
+            const uint32_t m3 = static_cast<uint32_t>(__popcll(mm));
+            // per candidate j: lb_j = pool entries below it; per pool entry:
+            // nlt = candidates above it, so it moves up by m3 - nlt
+            uint32_t lb = 0, rank = 0;
+            uint32_t nlt[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s) nlt[s] = 0;
+            while (mm) {
+                const int j = static_cast<int>(__builtin_ctzll(mm));
+                mm &= mm - 1;
+                const uint64_t kj = readlane64(key, j);
+                uint32_t below = 0;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const bool lt = P[s] < kj;
+                    below += static_cast<uint32_t>(__popcll(__ballot(lt)));
+                    nlt[s] += lt ? 1u : 0u;
+                }
+                lb = static_cast<int>(lane) == j ? below : lb;
+                rank += kj < key ? 1u : 0u;
+            }
+            const uint32_t nsize = min(size + m3, ef);
+            // merge through LDS: each entry and candidate goes to its new
+            // position (those that fall off to the spare slot EF_CAP); every
+            // position below nsize is written, those above still hold the
+            // empty entries they were given at the start
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const uint32_t np = static_cast<uint32_t>(s * kWave) + lane + m3 - nlt[s];
+                const uint32_t dst = np < nsize ? np : static_cast<uint32_t>(EF_CAP);
+                w.st[dst] = make_uint4(static_cast<uint32_t>(P[s]), static_cast<uint32_t>(P[s] >> 32),
+                                       I[s], 0u);
+            }
+            {
+                const uint32_t np = lb + rank;
+                const uint32_t dst = (cand && np < nsize) ? np : static_cast<uint32_t>(EF_CAP);
+                w.st[dst] = make_uint4(static_cast<uint32_t>(key), static_cast<uint32_t>(key >> 32),
+                                       nid, 0u);
+            }
+            wave_sync();
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const uint4 e = w.st[static_cast<uint32_t>(s * kWave) + lane];
+                P[s] = (static_cast<uint64_t>(e.y) << 32) | e.x;
+                I[s] = e.z;
+            }
+            size = nsize;
+            wave_sync();
+        }
+        seq += deg;
+        // res_pool.insert (qg.hpp:282)
+        rb_insert<KS>(rb, kq, cur, sqr_y, lane);
+    }
+
+    // update_results (qg.hpp:395-418): fewer than k nodes expanded.  The ids
+    // snapshot holds k + 1 entries, those past the buffer's size 0 (the
+    // ResultBuffer is zero-initialised, buffer.hpp:81-102), so node 0 may be
+    // probed.
+    if (!failed && kq > 0 && rb.size < kq) {
+        uint32_t snap[KS];
+#pragma unroll
+        for (int s = 0; s < KS; ++s) snap[s] = rb.ri[s];
+        const uint32_t snap_size = rb.size;
+        for (uint32_t t = 0; t <= kq && !failed; ++t) {
+            const uint32_t node = t < snap_size ? rb_id<KS>(snap, t) : 0u;
+            const unsigned char* rec = ix.rec + static_cast<size_t>(node) * ix.stride;
+            const uint32_t* nb = reinterpret_cast<const uint32_t*>(rec + ix.off_ids);
+            for (uint32_t i = 0; i < deg; ++i) {
+                const uint32_t x = uniform(nb[i]);
+                const uint32_t xs = v_test_set(vis, x, lane);
+                if (xs == kVisSeen) continue;
+                if (xs == kVisFull) {
+                    if (!GLOBAL && job.ovf) return h2_defer(job, q, lane);
+                    failed = true;
+                    break;
+                }
+                const float* row =
+                    reinterpret_cast<const float*>(ix.rec + static_cast<size_t>(x) * ix.stride);
+                float xr[kRegJ];
+                load_partials(xr, row, dim, lane);
+                rb_insert<KS>(rb, kq, x, exact_l2_regs(xr, qr, row, w, dim, lane), lane);
+            }
+            if (rb.size == kq) break;
+        }
+    }
+
+    // copy_results (buffer.hpp:102): unfilled slots hold id 0; a query whose
+    // visited set overflowed returns no ids (0xFFFFFFFF everywhere)
+    uint32_t* oi = job.out_ids + static_cast<size_t>(q) * job.kstride;
+    float* od = job.out_d ? job.out_d + static_cast<size_t>(q) * job.kstride : nullptr;
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+        const uint32_t j = static_cast<uint32_t>(s * kWave) + lane;
+        if (j < job.kstride) {
+            uint32_t id = kInvalid;
+            float d = INFINITY;
+            if (j < kq && !failed) {
+                id = j < rb.size ? rb.ri[s] : 0u;
+                if (j < rb.size) d = rb.rd[s];
+            }
+            oi[j] = id;
+            if (od) od[j] = d;
+        }
+    }
+    if (lane == 0) {
+        if (job.out_stats) {
+            job.out_stats[2 * static_cast<size_t>(q)] = failed ? kInvalid : n_exp;
+            job.out_stats[2 * static_cast<size_t>(q) + 1] = n_pop;
+        }
+        if (failed) atomicOr(job.err, 1u);
+    }
+    if (job.done) signal_host(job.done + q, lane, failed ? kFlagFailed : kFlagDone);
+    wave_sync();
+    return true;
+}
+
+template <int EF_CAP, int CW, int KS>
+__device__ __forceinline__ void h2_search_body(const H2Dev& ix, const H2Job& job,
+                                               uint32_t hash_bits, const Layout& lay) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wid = threadIdx.x / kWave;
+    const H2Lds w = carve(lds + static_cast<size_t>(wid) * lay.wave_bytes, lay);
+    const uint32_t gwave = blockIdx.x * (blockDim.x / kWave) + wid;
+    // wave gwave first takes query gwave, then sw + atomicAdd(counter) (sw =
+    // waves in the grid), so the launch does not start with one atomic per
+    // wave on a single counter
+    const uint32_t sw = gridDim.x * (blockDim.x / kWave);
+    uint32_t q = gwave;
+    for (;;) {
+        q = uniform(q);
+        if (q >= job.nq) break;
+        (void)h2_search_one<EF_CAP, CW, KS, false>(ix, job, q, w, hash_bits, nullptr, lane);
+        uint32_t nx = 0;
+        if (lane == 0) nx = sw + atomicAdd(job.counter, 1u);
+        q = __shfl(nx, 0);
+    }
+}
+
+// The overflow kernel: searches again, with the visited set in a per-wave
+// device table, the queries the search kernel put on the overflow list
+// (run after it on the same stream; it finds an empty list at once).
+template <int EF_CAP, int CW, int KS>
+__global__ void __launch_bounds__(64) h2_overflow_kernel(H2Dev ix, H2Job job, uint32_t hash_bits,
+                                                          Layout lay) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const H2Lds w = carve(lds, lay);
+    const uint32_t n = uniform(__hip_atomic_load(&job.ovf[0], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT));
+    uint32_t* gtab = job.gvis ? job.gvis + (static_cast<size_t>(blockIdx.x) << job.gvis_bits)
+                              : nullptr;
+    for (;;) {
+        uint32_t i = 0;
+        if (lane == 0) i = atomicAdd(&job.ovf[1], 1u);
+        i = uniform(__shfl(i, 0));
+        if (i >= n) break;
+        const uint32_t q = uniform(job.ovf[2 + i]);
+        (void)h2_search_one<EF_CAP, CW, KS, true>(ix, job, q, w, hash_bits, gtab, lane);
+    }
+}
+
+// Occupancy variants: uncapped registers, and capped for 6 or 8 waves per
+// SIMD.
+template <int EF_CAP, int CW, int KS>
+__global__ void __launch_bounds__(256) h2_search_kernel(H2Dev ix, H2Job job, uint32_t hash_bits,
+                                                         Layout lay) {
+    h2_search_body<EF_CAP, CW, KS>(ix, job, hash_bits, lay);
+}
+
+template <int EF_CAP, int CW, int KS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 6)))
+h2_search_kernel_w6(H2Dev ix, H2Job job, uint32_t hash_bits, Layout lay) {
+    h2_search_body<EF_CAP, CW, KS>(ix, job, hash_bits, lay);
+}
+
+template <int EF_CAP, int CW, int KS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8)))
+h2_search_kernel_w8(H2Dev ix, H2Job job, uint32_t hash_bits, Layout lay) {
+    h2_search_body<EF_CAP, CW, KS>(ix, job, hash_bits, lay);
+}
+
+}  // namespace
+}  // namespace lqg

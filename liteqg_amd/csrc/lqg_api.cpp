@@ -577,8 +577,8 @@ lqg_status lqg_set_launch(lqg_index* h, uint32_t ef, const lqg_launch* cfg) {
     if (cfg && !fs && cfg->occupancy != -1 && cfg->occupancy != 0 && cfg->occupancy != 6 &&
         cfg->occupancy != 8)
         return fail(LQG_ERR_ARG, "SQ8 occupancy must be -1, 0, 6 or 8");
-    if (cfg && fs && cfg->occupancy != 0 && cfg->occupancy != 5 && cfg->occupancy != 6)
-        return fail(LQG_ERR_ARG, "FastScan occupancy must be 0, 5 or 6");
+    if (cfg && fs && cfg->occupancy != 0 && cfg->occupancy != 6)
+        return fail(LQG_ERR_ARG, "FastScan occupancy must be 0 or 6");
     if (cfg && fs && cfg->occupancy != 0 && h->h2_ready && !lqg::h2_occupancy_ok(h->h2dev()))
         return fail(LQG_ERR_UNSUPPORTED,
                     "FastScan occupancy caps need at most 4 code words per lane (P / lanes per "
@@ -661,16 +661,17 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
         float best_ms = 1e30f;
         lqg::H2Cfg best_cfg = base;
         std::vector<lqg::H2Cfg> cands;
-        const int n_occ = h->h2_ready && lqg::h2_occupancy_ok(h->h2dev()) ? 3 : 1;
-        for (int oi = 0; oi < n_occ; ++oi)
+        for (int occ : {0, 6}) {
+            if (!h->h2_ready || !lqg::h2_variant_ok(h->h2dev(), occ, k)) continue;
             for (int extra = -1; extra < 2; ++extra)
                 for (int wg : {1, 2, 4}) {
                     lqg::H2Cfg c = base;
-                    c.occupancy = oi == 0 ? 0 : oi + 4;  // 0, 5, 6
+                    c.occupancy = occ;
                     c.waves_per_group = wg;
                     c.hash_bits = base.hash_bits + extra;
                     if (c.hash_bits >= 6) cands.push_back(c);
                 }
+        }
         for (const lqg::H2Cfg& c : cands) {
             h->h2_launch[slot] = c;
             h->h2_launch_set[slot] = true;

@@ -181,8 +181,17 @@ lqg_status h2_batch_cfg(const lqg_index* h, uint64_t nq, uint32_t ef_max, uint32
                         H2Cfg* out) {
     const int cls = ef_class(ef_max);
     if (cls == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024 is not supported on the device");
+    if (!h2_kmax_ok(kmax))
+        return fail(LQG_ERR_UNSUPPORTED, "FastScan: k above 128 is not supported on the device");
     H2Cfg cfg = h2_effective_launch(h, cls);
     const H2Dev ix = h->h2dev();
+    if (!h2_variant_ok(ix, cfg.occupancy, kmax)) {
+        if (h->h2_launch_set[class_slot(cls)])
+            return fail(LQG_ERR_UNSUPPORTED,
+                        "FastScan: no kernel with this register cap for this index shape (the "
+                        "6-wave cap needs at most 4 code words per lane)");
+        cfg.occupancy = 0;
+    }
     if (nq < static_cast<uint64_t>(cfg.waves_per_group)) cfg.waves_per_group = 1;
     while (cfg.waves_per_group > 1 && h2_lds_bytes(ix, cfg, kmax) > 160 * 1024)
         cfg.waves_per_group /= 2;
@@ -195,16 +204,23 @@ lqg_status h2_batch_cfg(const lqg_index* h, uint64_t nq, uint32_t ef_max, uint32
     return LQG_OK;
 }
 
-// Device scratch of such a batch: per-wave overflow visited tables in device
-// memory (2^h2_gvis_bits entries each, none when that is 0 or the scratch
-// has no_overflow set), used once a query has expanded more nodes than half
-// its LDS table holds; the overflow flag and the work-queue head; the
-// GEMM-rotated queries.
+// Waves of the overflow kernel after a batch of nq queries.
+uint32_t overflow_waves(uint64_t nq) {
+    return static_cast<uint32_t>(std::min<uint64_t>(nq, kOverflowWaves));
+}
+
+// Device scratch of such a batch: the overflow list (nq + 2 words) and the
+// overflow kernel's per-wave visited tables in device memory (2^h2_gvis_bits
+// entries each), for queries whose set outgrows half their LDS table (none
+// when h2_gvis_bits is 0 or the scratch has no_overflow set: such queries
+// fail); the failure flag and the work-queue head; the GEMM-rotated queries.
 lqg_status h2_reserve(const lqg_index* h, const H2Cfg& cfg, uint64_t nq, Scratch* sc) {
+    (void)cfg;
     const uint32_t gbits = sc->no_overflow ? 0u : h->h2_gvis_bits;
-    if (gbits)
-        LQG_HIP_TRY(sc->gvis.reserve((static_cast<size_t>(cfg.groups) * cfg.waves_per_group)
-                                     << gbits));
+    if (gbits) {
+        LQG_HIP_TRY(sc->gvis.reserve(static_cast<size_t>(overflow_waves(nq)) << gbits));
+        LQG_HIP_TRY(sc->ovf.reserve(nq + 2));
+    }
     LQG_HIP_TRY(sc->err.reserve(1));
     LQG_HIP_TRY(sc->counter.reserve(1));
     if (h->h2_rotation == LQG_ROTATION_GEMM) LQG_HIP_TRY(sc->rot.reserve(nq * h->h2_p));
@@ -251,12 +267,15 @@ lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const
     job.counter = sc->counter.p;
     job.gvis = gbits ? sc->gvis.p : nullptr;
     job.gvis_bits = gbits;
+    job.ovf = gbits ? sc->ovf.p : nullptr;
     job.err = sc->err.p;
     job.done = sc->done;
     job.rot = h->h2_rotation == LQG_ROTATION_GEMM ? sc->rot.p : nullptr;
     LQG_HIP_TRY(launch_reset_counter(sc->counter.p, s));
     LQG_HIP_TRY(launch_reset_counter(sc->err.p, s));
+    if (job.ovf) LQG_HIP_TRY(launch_reset_counter(job.ovf, s, 2));
     LQG_HIP_TRY(h2_launch_search(ix, job, cfg, s));
+    if (job.ovf) LQG_HIP_TRY(h2_launch_overflow(ix, job, cfg, overflow_waves(nq), s));
     return LQG_OK;
 }
 

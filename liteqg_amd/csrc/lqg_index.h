@@ -53,7 +53,8 @@ struct DevBuf {
 struct Scratch {
     DevBuf<uint32_t> counter;  // work-queue head
     DevBuf<uint32_t> err;      // FastScan visited-set overflow flag
-    DevBuf<uint32_t> gvis;     // FastScan per-wave overflow visited tables
+    DevBuf<uint32_t> gvis;     // FastScan overflow kernel's per-wave visited tables
+    DevBuf<uint32_t> ovf;      // FastScan overflow list (count, counter, query indices)
     DevBuf<float> rot;         // FastScan queries rotated by the GEMM (nq x P)
     // Per-query completion flags the launch sets (host-coherent memory owned by
     // the caller, e.g. a serving lane), or null.
@@ -65,6 +66,7 @@ struct Scratch {
         counter.release();
         err.release();
         gvis.release();
+        ovf.release();
         rot.release();
     }
 };

@@ -100,6 +100,10 @@ struct H2Job {
     uint32_t* err;          // set to 1 if a visited set overflows both tables
     const float* rot = nullptr;  // nq x p queries rotated beforehand (GEMM), or null (FHT)
     uint32_t* done = nullptr;    // per-query completion flags, as in SearchJob
+    // Overflow list (2 + nq words: count, overflow-kernel work counter, query
+    // indices), zeroed before launch; null: a query whose LDS visited table
+    // would pass half full fails instead.
+    uint32_t* ovf = nullptr;
 };
 
 struct H2Cfg {
@@ -107,7 +111,7 @@ struct H2Cfg {
     int waves_per_group; // 1..4
     int groups;          // persistent grid size (0 = fill the device)
     int hash_bits;       // log2 entries of the per-wave LDS visited table
-    int occupancy = 0;   // 0 = uncapped registers, 5 or 6 = capped for that many waves per SIMD
+    int occupancy = 0;   // 0 = uncapped registers, 6 or 8 = capped for that many waves per SIMD
 };
 
 // Returns LDS bytes one workgroup needs for this configuration.
@@ -119,12 +123,22 @@ size_t h2_lds_bytes(const H2Dev& ix, const H2Cfg& cfg, uint32_t kmax);
 // wave per query, bit-identical to it), 1 = one FP32 GEMM on the matrix cores.
 hipError_t h2_launch_rotate_queries(const H2Dev& ix, const float* queries, uint32_t nq,
                                     int method, float* out, hipStream_t stream);
-// Whether the register-capped variants (occupancy 5, 6) exist for this index
+// Whether the register-capped variants (occupancy 6, 8) exist for this index
 // shape (at most 4 code words per lane).
 bool h2_occupancy_ok(const H2Dev& ix);
+// Whether the FastScan kernels take this kmax (k <= 256, in registers).
+bool h2_kmax_ok(uint32_t kmax);
+// Whether a kernel variant exists for this index shape, register cap and kmax.
+bool h2_variant_ok(const H2Dev& ix, int occupancy, uint32_t kmax);
 int h2_groups(const H2Dev& ix, const H2Cfg& cfg, uint32_t kmax, uint32_t nq);
 hipError_t h2_launch_search(const H2Dev& ix, const H2Job& job, const H2Cfg& cfg,
                             hipStream_t stream);
+// The overflow kernel after a search launch with job.ovf set: `groups`
+// one-wave workgroups, wave g with the device visited table at
+// job.gvis + (g << job.gvis_bits).
+constexpr uint32_t kOverflowWaves = 256;
+hipError_t h2_launch_overflow(const H2Dev& ix, const H2Job& job, const H2Cfg& cfg,
+                              uint32_t groups, hipStream_t stream);
 // Query preparation (and optionally one node's FastScan estimates) for each
 // query: lut (nq x 4p), params (nq x 3: lo, width, sumq bits), sqr_y (nq), est
 // (nq x deg); nodes may be null (no scan).

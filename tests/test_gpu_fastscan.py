@@ -159,10 +159,11 @@ def test_search_matches_oracle(oracle_mod, tmp_path, shape):
 @pytest.mark.parametrize("shape", [s for s in SHAPES if s[1] <= 256],
                          ids=lambda s: f"n{s[0]}_d{s[1]}_deg{s[2]}")
 def test_occupancy_variants_match_oracle(oracle_mod, tmp_path, shape):
-    """The register-capped kernels (5 and 6 waves per SIMD, code words per lane
-    <= 4) give the oracle's ids, distances and counts, also with a visited
-    table small enough to move to device memory; P/lanes above 128 bits
-    rejects the caps."""
+    """The uncapped and the register-capped kernel (6 waves per SIMD, code
+    words per lane <= 4) give the oracle's ids, distances and counts, also
+    with an LDS visited table small enough (64 entries at ef 256) that queries
+    go to the overflow kernel and its device table; P/lanes above 128 bits
+    rejects the cap."""
     from liteqg_amd._lib import LqgError
 
     O = oracle_mod
@@ -176,10 +177,10 @@ def test_occupancy_variants_match_oracle(oracle_mod, tmp_path, shape):
     cw = ix.padded_dim // 8 // (1 if deg >= 64 else 2) // 4
     if cw > 4:
         with pytest.raises(LqgError) as e:
-            ix.set_launch(0, waves_per_group=2, cache_bits=10, occupancy=5)
+            ix.set_launch(0, waves_per_group=2, cache_bits=10, occupancy=6)
         assert e.value.status == 6
         return
-    for occ in (5, 6):
+    for occ in (0, 6):
         for ef, k, bits in ((64, 10, 10), (150, 10, 10), (200, 20, 10), (256, 10, 6)):
             ix.set_launch(0, waves_per_group=4, cache_bits=bits, occupancy=occ)
             assert ix.get_launch(ef)["occupancy"] == occ
