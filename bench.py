@@ -757,7 +757,7 @@ def main():
             ix.set_launch(ef, *v)
         else:
             ix.autotune(Q, ef, k)
-        ms, _ = time_launches(db, ef, args.probe_launches, 0)
+        ms, _ = time_launches(db, ef, args.probe_launches, 2)
         x = torch.zeros(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
         y.copy_(x)  # reads 1 GiB, writes 1 GiB

@@ -246,9 +246,9 @@ typedef struct {
                                 waves/SIMD, 128 VGPRs, for pool classes up to 256;
                                 uncapped above), -1 = uncapped, 8 = 64 VGPRs
                                 (8 waves/SIMD), 6 = 80 VGPRs (6 waves/SIMD).
-                                FastScan: 0 = uncapped, 6 waves/SIMD (80 VGPRs),
-                                for indexes with P / lanes per neighbour <= 128
-                                code bits                                        */
+                                FastScan: 0 = uncapped, 6 or 8 waves/SIMD (80 or
+                                64 VGPRs), for indexes with P / lanes per
+                                neighbour <= 128 code bits (8: k <= 64)          */
     int32_t groups;          /* workgroups of the persistent grid; 0 = fill the device  */
 } lqg_launch;
 

@@ -1,8 +1,8 @@
 // h2_kern_variants.h -- the table of FastScan search-kernel variants one
 // instantiation file compiles (h2_kern_ks1.hip, h2_kern_ks2.hip): pool
 // capacity class x code words per lane x register cap, for one result-buffer
-// size.  Register caps (6 waves per SIMD) exist for at most 4 code words per
-// lane.
+// size.  Register caps (6 or 8 waves per SIMD) exist for at most 4 code words
+// per lane; the 8-wave cap only for k <= 64.
 #pragma once
 
 #include "h2_kernel.h"
@@ -20,6 +20,16 @@ SearchFn variant_for_cw(int cw, int occ) {
             case 2: return &h2_search_kernel_w6<EF_CAP, 2, KS>;
             case 4: return &h2_search_kernel_w6<EF_CAP, 4, KS>;
             default: return nullptr;
+        }
+    }
+    if constexpr (KS == 1) {
+        if (occ == 8) {
+            switch (cw) {
+                case 1: return &h2_search_kernel_w8<EF_CAP, 1, KS>;
+                case 2: return &h2_search_kernel_w8<EF_CAP, 2, KS>;
+                case 4: return &h2_search_kernel_w8<EF_CAP, 4, KS>;
+                default: return nullptr;
+            }
         }
     }
     if (occ != 0) return nullptr;

@@ -61,7 +61,7 @@ __host__ __device__ inline Layout h2_layout(uint32_t dim, uint32_t p, uint32_t e
     uint32_t o = 0;
     l.has_qv = with_qv ? 1 : 0;
     l.qv = o;   o += with_qv ? align16(4 * dim) : 0u;
-    l.lut = o;  o += align16(4 * p + 16);
+    l.lut = o;  o += align16(2u * p + 256u * ((p / 8u + 15u) / 16u));
     // one region for the preparation buffers (tb, qb) and, once the LUT is
     // built, the pool merge staging (ef_cap records and a spare)
     const uint32_t prep = align16(4 * p) + align16(p);
@@ -84,8 +84,10 @@ constexpr float kC63 = 1.0f / kInv63;   // 1/width = fl(1/fl(1/63)) / (hi - lo)
 // Estimate bits that sort like the floats (-0 taken as +0, as a float
 // comparison does).
 __device__ __forceinline__ uint32_t sortable(float x) {
-    const uint32_t u = __float_as_uint(x == 0.0f ? 0.0f : x);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    // x + 0 turns -0 into +0 and leaves every other value as it is
+    const uint32_t u = __float_as_uint(x + 0.0f);
+    const uint32_t m = static_cast<uint32_t>(static_cast<int32_t>(u) >> 31);
+    return u ^ (m | 0x80000000u);
 }
 
 __device__ __forceinline__ uint64_t sb_key(float est, uint32_t seq) {
@@ -115,6 +117,21 @@ __device__ __forceinline__ H2Lds carve(unsigned char* base, const Layout& l) {
 }
 
 // ---------------------------------------------------------------------------
+// LUT layout for the scan.  A code byte holds codebook 2g (low nibble) and
+// 2g + 1 (high nibble) of codebook pair g.  Even codebooks: 16 bytes per pair,
+// row after row (pair g's entry j at 16g + j).  Odd codebooks: column g % 16
+// of a 16 x 16 block per 16 pairs (entry j at 2p + 256 (g / 16) + 16 j +
+// g % 16), so that the high nibble, masked in place (16 j), is the offset.
+// 4p bytes (2p + 256 for p = 64).
+__host__ __device__ inline uint32_t lut_pos(uint32_t p, uint32_t c, uint32_t j) {
+    const uint32_t g = c >> 1;
+    return (c & 1u) ? 2u * p + 256u * (g >> 4) + 16u * j + (g & 15u) : 16u * g + j;
+}
+
+__host__ __device__ inline uint32_t lut_bytes(uint32_t p) {
+    return 2u * p + 256u * ((p / 8u + 15u) / 16u);
+}
+
 // Query preparation (qg_query.hpp:28-43) into LDS: qv (raw query), lut.
 struct Prep {
     float lo, width;
@@ -181,17 +198,17 @@ __device__ Prep prepare(const H2Dev& ix, const float* qsrc, const float* rsrc, c
     for (int o = 32; o > 0; o >>= 1) sq += __shfl_xor(sq, o);
     wave_sync();
     // pack_lut_impl (fastscan_impl.hpp:258-270): entry j of codebook c is the
-    // uint8 sum of q[4c + 3 - b] over the set bits b of j.  With two lanes per
-    // neighbour the second half of the codebooks sits 16 bytes further, so the
-    // two halves of a wavefront read different LDS banks.
-    const uint32_t ncb = p / 4, cpl = ncb / lpv;
+    // uint8 sum of q[4c + 3 - b] over the set bits b of j.  Stored at
+    // lut_pos(p, c, j) (the scan's layout, lut_pos).
+    const uint32_t ncb = p / 4;
+    (void)lpv;
     for (uint32_t idx = lane; idx < 16u * ncb; idx += kWave) {
         const uint32_t c = idx >> 4, j = idx & 15u;
         uint32_t s = 0;
 #pragma unroll
         for (uint32_t b = 0; b < 4; ++b)
             if ((j >> b) & 1u) s += w.qb[4 * c + 3 - b];
-        w.lut[16 * c + (c >= cpl ? 16u : 0u) + j] = static_cast<uint8_t>(s & 0xFFu);
+        w.lut[lut_pos(p, c, j)] = static_cast<uint8_t>(s & 0xFFu);
     }
     wave_sync();
     Prep r;
@@ -214,10 +231,11 @@ constexpr int kRegJ = 8;
 __device__ __forceinline__ void load_partials(float (&r)[kRegJ], const float* src, uint32_t dim,
                                               uint32_t lane) {
     const uint32_t nj = dim / 16, p = lane & 15u;
+    const gptr<float> g = uniform_gptr<float>(src);
 #pragma unroll
     for (int u = 0; u < kRegJ; ++u) {
         r[u] = 0.0f;
-        if (static_cast<uint32_t>(u) < nj) r[u] = src[p + 16u * static_cast<uint32_t>(u)];
+        if (static_cast<uint32_t>(u) < nj) r[u] = g[p + 16u * static_cast<uint32_t>(u)];
     }
 }
 
@@ -430,32 +448,83 @@ __device__ __forceinline__ uint32_t rb_id(const uint32_t (&ri)[KS], uint32_t t) 
 }
 
 // ---------------------------------------------------------------------------
-// FastScan sum of one lane: CW code words (8 codebooks each) starting at
-// codebook c0, LUT base already offset for the lane's half.
+// base + byte J of t in one instruction (SDWA source select).
+template <int J>
+__device__ __forceinline__ uint32_t add_byte(uint32_t base, uint32_t t) {
+    uint32_t r;
+    if constexpr (J == 0)
+        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD "
+            "src1_sel:BYTE_0" : "=v"(r) : "v"(base), "v"(t));
+    else if constexpr (J == 1)
+        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD "
+            "src1_sel:BYTE_1" : "=v"(r) : "v"(base), "v"(t));
+    else if constexpr (J == 2)
+        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD "
+            "src1_sel:BYTE_2" : "=v"(r) : "v"(base), "v"(t));
+    else
+        asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD "
+            "src1_sel:BYTE_3" : "=v"(r) : "v"(base), "v"(t));
+    return r;
+}
+
+typedef __attribute__((address_space(3))) const uint8_t lds_u8;
+
+// The two LDS base addresses of a lane's LUT lookups (lut_pos for the lane's
+// first codebook pair, 4 CW pairs per lane, `part` of lpv lanes per vector).
+struct LutBase {
+    uint32_t even, odd;
+};
+
 template <int CW>
-__device__ __forceinline__ uint32_t scan_sum(const uint32_t (&cw)[CW], const uint8_t* lut) {
+__device__ __forceinline__ LutBase lut_base(const uint8_t* lut, uint32_t p, uint32_t part) {
+    // the LDS offset of the generic pointer (an address-space cast)
+    const uint32_t l = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((lds_u8*)lut));
+    const uint32_t g0 = 4u * CW * part;
+    return LutBase{l + 16u * g0, l + 2u * p + 256u * (g0 >> 4) + (g0 & 15u)};
+}
+
+// FastScan sum of one lane: CW code words, pairs g0 .. g0 + 4 CW - 1 of the
+// lane's LUT bases: per word, the low nibbles masked in place index the even
+// rows, the high nibbles masked in place (16 j) the odd columns.
+template <int CW>
+__device__ __forceinline__ uint32_t scan_sum(const uint32_t (&cw)[CW], LutBase lb) {
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < CW; ++k) {
-        const uint32_t word = cw[k];
-#pragma unroll
-        for (int n = 0; n < 8; ++n) s += lut[16 * (8 * k + n) + ((word >> (4 * n)) & 15u)];
+        const uint32_t t = cw[k] & 0x0F0F0F0Fu, v = cw[k] & 0xF0F0F0F0u;
+#define LQG_SCAN_BYTE(J)                                                                     \
+    {                                                                                       \
+        const uint32_t b = 4u * static_cast<uint32_t>(k) + (J);                             \
+        s += reinterpret_cast<lds_u8*>(static_cast<uintptr_t>(add_byte<J>(lb.even, t)))[16u * b]; \
+        s += reinterpret_cast<lds_u8*>(static_cast<uintptr_t>(add_byte<J>(lb.odd, v)))     \
+            [256u * (b >> 4) + (b & 15u)];                                                  \
+    }
+        LQG_SCAN_BYTE(0)
+        LQG_SCAN_BYTE(1)
+        LQG_SCAN_BYTE(2)
+        LQG_SCAN_BYTE(3)
+#undef LQG_SCAN_BYTE
     }
     return s;
 }
 
+// CW code words from word index w of a global array (w a multiple of
+// min(CW, 4): the loads are 4, 8 or 16 bytes wide and aligned).
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 template <int CW>
-__device__ __forceinline__ void load_codes(uint32_t (&cw)[CW], const uint8_t* src) {
+__device__ __forceinline__ void load_codes(uint32_t (&cw)[CW], gptr<uint32_t> base, uint32_t w) {
     if constexpr (CW == 1) {
-        cw[0] = *reinterpret_cast<const uint32_t*>(src);
+        cw[0] = base[w];
     } else if constexpr (CW == 2) {
-        const uint2 v = *reinterpret_cast<const uint2*>(src);
+        const u32x2 v = ((gptr<u32x2>)base)[w / 2];
         cw[0] = v.x;
         cw[1] = v.y;
     } else {
 #pragma unroll
         for (int k = 0; k < CW; k += 4) {
-            const uint4 v = *reinterpret_cast<const uint4*>(src + 4 * k);
+            const u32x4 v = ((gptr<u32x4>)base)[w / 4 + k / 4];
             cw[k] = v.x;
             cw[k + 1] = v.y;
             cw[k + 2] = v.z;
@@ -468,10 +537,10 @@ __device__ __forceinline__ void load_codes(uint32_t (&cw)[CW], const uint8_t* sr
 // neighbours starting at vb).  Returns the estimate on every lane of the
 // neighbour's lanes.
 template <int CW>
-__device__ __forceinline__ float lane_estimate(const uint32_t (&cw)[CW], const uint8_t* lut_part,
+__device__ __forceinline__ float lane_estimate(const uint32_t (&cw)[CW], LutBase lb,
                                                uint32_t lpv, const Prep& pr, float sqr_y,
                                                float tx, float dq, float vq) {
-    uint32_t s = scan_sum<CW>(cw, lut_part);
+    uint32_t s = scan_sum<CW>(cw, lb);
     // the two lanes of a neighbour: DPP quad_perm [1, 0, 3, 2]
     if (lpv == 2)
         s += static_cast<uint32_t>(
@@ -510,10 +579,9 @@ __device__ bool h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
     const uint32_t deg = ix.deg, dim = ix.dim;
     const uint32_t lpv = deg >= 64 ? 1u : 2u;
     const uint32_t vp = kWave / lpv;              // neighbours per pass
-    const uint32_t cpl = (ix.p / 4) / lpv;        // codebooks per lane
     const uint32_t code_bytes = ix.p / 8;         // per neighbour
     const uint32_t part = lane % lpv;
-    const uint8_t* lut_part = w.lut + 16u * cpl * part + (part ? 16u : 0u);
+    const LutBase lutb = lut_base<CW>(w.lut, ix.p, part);
 
     const Prep pr = prepare(ix, job.queries + static_cast<size_t>(q) * dim,
                             job.rot ? job.rot + static_cast<size_t>(q) * ix.p : nullptr, w, lpv,
@@ -554,9 +622,11 @@ __device__ bool h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
     float qr[kRegJ];
     load_partials(qr, job.queries + static_cast<size_t>(q) * dim, dim, lane);
     // per-lane offsets inside a node record for the neighbour this lane scans
-    // in a pass (the pass's first neighbour added per pass)
+    // in a pass (the pass's first neighbour added per pass); every load of a
+    // pass is then a uniform base plus one of these
     const uint32_t lv = lane / lpv;
     const uint32_t code_off = ix.off_codes + lv * code_bytes + part * (code_bytes / lpv);
+    const uint32_t lv4 = 4u * lv, lv2 = 2u * lv;
 
     while (!failed) {
         // pop: first unchecked entry (buffer.hpp:52-60)
@@ -600,25 +670,27 @@ __device__ bool h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
         float sqr_y = 0.0f;
 
         for (uint32_t vb = 0; vb < deg; vb += vp) {
-            // the pass's loads first (codes, factors, id); lanes past the last
-            // neighbour (a pass of a degree that is not a multiple of the
-            // pass width) read the last one and take no part
+            // the pass's loads first (codes, factors, id).  Lanes past the
+            // last neighbour (a pass of a degree that is not a multiple of
+            // the pass width) read bytes further on in the record, or in the
+            // spare record after the last one, and take no part.
             const uint32_t vi = vb + lv;
             const bool act = vi < deg;
-            const uint32_t vr = act ? vi : deg - 1u;
             uint32_t cw[CW];
-            load_codes<CW>(cw, rec + code_off + (vr - lv) * code_bytes);
-            const uint32_t nid = ids[vr];
-            const float tx = fac[vr];
-            const float dq = fac[deg + vr];
-            const float vq = fac[2 * deg + vr];
+            load_codes<CW>(cw, uniform_gptr<uint32_t>(rec + vb * code_bytes), code_off / 4u);
+            const uint32_t nid = uniform_gptr<uint32_t>(ids + vb)[lv];
+            const float tx = uniform_gptr<float>(fac + vb)[lv];
+            const float dq = uniform_gptr<float>(fac + deg + vb)[lv];
+            const float vq = uniform_gptr<float>(fac + 2 * deg + vb)[lv];
             // scan_neighbors (qg.hpp:355): exact distance to the node
             if (vb == 0) sqr_y = exact_l2_regs(xr, qr, raw, w, dim, lane);
-            const float est = lane_estimate<CW>(cw, lut_part, lpv, pr, sqr_y, tx, dq, vq);
+            const float est = lane_estimate<CW>(cw, lutb, lpv, pr, sqr_y, tx, dq, vq);
             // is_full / visited_.get (qg.hpp:375-377); candidate i gets insertion
             // number seq + i.  The visited table is probed only by the lanes the
             // pool would take.
-            const uint64_t key = sb_key(est, seq + vi);
+            // sb_key(est, seq + vi), the low word a uniform base minus 2 lv
+            const uint32_t klo = ((0x7FFFFFFFu - seq - vb) << 1) - lv2;
+            const uint64_t key = (static_cast<uint64_t>(sortable(est)) << 32) | klo;
             bool cand = act && part == 0;
             if (size == ef) {
                 const uint32_t t = size - 1u;

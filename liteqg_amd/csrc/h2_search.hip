@@ -61,13 +61,9 @@ __global__ void __launch_bounds__(64) h2_probe_kernel(H2Dev ix, const float* que
     const uint32_t deg = ix.deg;
     const uint32_t lpv = deg >= 64 ? 1u : 2u;
     const Prep pr = prepare(ix, queries + static_cast<size_t>(q) * ix.dim, nullptr, w, lpv, lane);
-    const uint32_t cpl = (ix.p / 4) / lpv;
-    // LUT in the reference's layout (no bank skew)
-    for (uint32_t idx = lane; idx < 4 * ix.p; idx += kWave) {
-        const uint32_t c = idx >> 4;
-        lut_out[static_cast<size_t>(q) * 4 * ix.p + idx] =
-            w.lut[idx + (lpv == 2 && c >= cpl ? 16u : 0u)];
-    }
+    // LUT in the reference's layout (entry j of codebook c at 16 c + j)
+    for (uint32_t idx = lane; idx < 4 * ix.p; idx += kWave)
+        lut_out[static_cast<size_t>(q) * 4 * ix.p + idx] = w.lut[lut_pos(ix.p, idx >> 4, idx & 15u)];
     if (lane == 0) {
         params[3 * q] = pr.lo;
         params[3 * q + 1] = pr.width;
@@ -79,7 +75,7 @@ __global__ void __launch_bounds__(64) h2_probe_kernel(H2Dev ix, const float* que
     const float sqr_y = exact_l2(reinterpret_cast<const float*>(rec), w, ix.dim, lane);
     if (lane == 0) sqr_out[q] = sqr_y;
     const uint32_t part = lane % lpv;
-    const uint8_t* lut_part = w.lut + 16u * cpl * part + (part ? 16u : 0u);
+    const LutBase lutb = lut_base<CW>(w.lut, ix.p, part);
     const float* fac = reinterpret_cast<const float*>(rec + ix.off_fac);
     const uint32_t code_bytes = ix.p / 8;
     for (uint32_t vb = 0; vb < deg; vb += kWave / lpv) {
@@ -88,8 +84,8 @@ __global__ void __launch_bounds__(64) h2_probe_kernel(H2Dev ix, const float* que
         uint32_t cw[CW];
         float tx = 0.0f, dq = 0.0f, vq = 0.0f;
         if (act) {
-            load_codes<CW>(cw, rec + ix.off_codes + static_cast<size_t>(vi) * code_bytes +
-                                   static_cast<size_t>(part) * (code_bytes / lpv));
+            load_codes<CW>(cw, uniform_gptr<uint32_t>(rec),
+                           (ix.off_codes + vi * code_bytes + part * (code_bytes / lpv)) / 4u);
             tx = fac[vi];
             dq = fac[deg + vi];
             vq = fac[2 * deg + vi];
@@ -97,7 +93,7 @@ __global__ void __launch_bounds__(64) h2_probe_kernel(H2Dev ix, const float* que
 #pragma unroll
             for (int k = 0; k < CW; ++k) cw[k] = 0;
         }
-        const float est = lane_estimate<CW>(cw, lut_part, lpv, pr, sqr_y, tx, dq, vq);
+        const float est = lane_estimate<CW>(cw, lutb, lpv, pr, sqr_y, tx, dq, vq);
         if (act && part == 0) est_out[static_cast<size_t>(q) * deg + vi] = est;
     }
 }

@@ -577,8 +577,8 @@ lqg_status lqg_set_launch(lqg_index* h, uint32_t ef, const lqg_launch* cfg) {
     if (cfg && !fs && cfg->occupancy != -1 && cfg->occupancy != 0 && cfg->occupancy != 6 &&
         cfg->occupancy != 8)
         return fail(LQG_ERR_ARG, "SQ8 occupancy must be -1, 0, 6 or 8");
-    if (cfg && fs && cfg->occupancy != 0 && cfg->occupancy != 6)
-        return fail(LQG_ERR_ARG, "FastScan occupancy must be 0 or 6");
+    if (cfg && fs && cfg->occupancy != 0 && cfg->occupancy != 6 && cfg->occupancy != 8)
+        return fail(LQG_ERR_ARG, "FastScan occupancy must be 0, 6 or 8");
     if (cfg && fs && cfg->occupancy != 0 && h->h2_ready && !lqg::h2_occupancy_ok(h->h2dev()))
         return fail(LQG_ERR_UNSUPPORTED,
                     "FastScan occupancy caps need at most 4 code words per lane (P / lanes per "
@@ -661,7 +661,7 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
         float best_ms = 1e30f;
         lqg::H2Cfg best_cfg = base;
         std::vector<lqg::H2Cfg> cands;
-        for (int occ : {0, 6}) {
+        for (int occ : {0, 6, 8}) {
             if (!h->h2_ready || !lqg::h2_variant_ok(h->h2dev(), occ, k)) continue;
             for (int extra = -1; extra < 2; ++extra)
                 for (int wg : {1, 2, 4}) {

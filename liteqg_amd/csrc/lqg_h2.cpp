@@ -76,7 +76,11 @@ lqg_status alloc_records(lqg_index* h) {
     if (st != LQG_OK) return st;
     h->h2_ready = false;
     ++h->generation;
-    LQG_HIP_TRY(h->d_h2rec.reserve(h->n * static_cast<size_t>(h->h2_stride)));
+    // one spare record after the last: the last pass of a degree that is
+    // not a multiple of the pass width reads past the neighbour lists
+    LQG_HIP_TRY(h->d_h2rec.reserve((h->n + 1) * static_cast<size_t>(h->h2_stride)));
+    LQG_HIP_TRY(hipMemset(h->d_h2rec.p + h->n * static_cast<size_t>(h->h2_stride), 0,
+                          h->h2_stride));
     LQG_HIP_TRY(h->d_h2signs.reserve(h->h2_p));
     st = host_rows(h, 0, h->raw.data(), 4 * h->dim, true);
     if (st != LQG_OK) return st;
@@ -188,8 +192,8 @@ lqg_status h2_batch_cfg(const lqg_index* h, uint64_t nq, uint32_t ef_max, uint32
     if (!h2_variant_ok(ix, cfg.occupancy, kmax)) {
         if (h->h2_launch_set[class_slot(cls)])
             return fail(LQG_ERR_UNSUPPORTED,
-                        "FastScan: no kernel with this register cap for this index shape (the "
-                        "6-wave cap needs at most 4 code words per lane)");
+                        "FastScan: no kernel with this register cap for this index shape and k "
+                        "(the caps need at most 4 code words per lane; 8 waves needs k <= 64)");
         cfg.occupancy = 0;
     }
     if (nq < static_cast<uint64_t>(cfg.waves_per_group)) cfg.waves_per_group = 1;

@@ -42,6 +42,21 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) {
     return static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
 }
 
+// A global-memory pointer the caller knows to be equal on every lane, in
+// scalar registers and typed as global memory, so that a load from it at a
+// per-lane 32-bit index takes the scalar-base addressing form (no 64-bit
+// address arithmetic per lane).
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T*;
+
+template <typename T>
+__device__ __forceinline__ gptr<T> uniform_gptr(const void* p) {
+    const uint64_t a = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = uniform(static_cast<uint32_t>(a));
+    const uint32_t hi = uniform(static_cast<uint32_t>(a >> 32));
+    return (gptr<T>)((static_cast<uint64_t>(hi) << 32) | lo);
+}
+
 // Sets flag to `value` once every lane's earlier writes are visible to the
 // host (system-scope fence on every lane, then one store).
 __device__ __forceinline__ void signal_host(uint32_t* flag, uint32_t lane,

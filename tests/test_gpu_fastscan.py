@@ -159,8 +159,8 @@ def test_search_matches_oracle(oracle_mod, tmp_path, shape):
 @pytest.mark.parametrize("shape", [s for s in SHAPES if s[1] <= 256],
                          ids=lambda s: f"n{s[0]}_d{s[1]}_deg{s[2]}")
 def test_occupancy_variants_match_oracle(oracle_mod, tmp_path, shape):
-    """The uncapped and the register-capped kernel (6 waves per SIMD, code
-    words per lane <= 4) give the oracle's ids, distances and counts, also
+    """The uncapped and the register-capped kernels (6 and 8 waves per SIMD,
+    code words per lane <= 4) give the oracle's ids, distances and counts, also
     with an LDS visited table small enough (64 entries at ef 256) that queries
     go to the overflow kernel and its device table; P/lanes above 128 bits
     rejects the cap."""
@@ -180,7 +180,7 @@ def test_occupancy_variants_match_oracle(oracle_mod, tmp_path, shape):
             ix.set_launch(0, waves_per_group=2, cache_bits=10, occupancy=6)
         assert e.value.status == 6
         return
-    for occ in (0, 6):
+    for occ in (0, 6, 8):
         for ef, k, bits in ((64, 10, 10), (150, 10, 10), (200, 20, 10), (256, 10, 6)):
             ix.set_launch(0, waves_per_group=4, cache_bits=bits, occupancy=occ)
             assert ix.get_launch(ef)["occupancy"] == occ
@@ -190,7 +190,7 @@ def test_occupancy_variants_match_oracle(oracle_mod, tmp_path, shape):
             assert np.array_equal(_bits(dis), _bits(e_dis)), (occ, ef)
             assert st["expansions"] == int(e_st[:, 0].sum())
     with pytest.raises(LqgError) as e:
-        ix.set_launch(0, waves_per_group=2, cache_bits=10, occupancy=8)
+        ix.set_launch(0, waves_per_group=2, cache_bits=10, occupancy=5)
     assert e.value.status == 1
     ix.reset_launch(0)
 
