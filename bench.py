@@ -758,6 +758,12 @@ def main():
         else:
             ix.autotune(Q, ef, k)
         ms, _ = time_launches(db, ef, args.probe_launches, 2)
+        db.launch(ef, with_stats=True)
+        torch.cuda.synchronize()
+        n_exp, n_pop = db.counters()
+        pct = np.percentile(n_exp, [50, 90, 99, 100]).astype(int).tolist()
+        log(f"probe counters: {n_exp.mean():.2f} expansions, {n_pop.mean():.2f} pops per query "
+            f"(nq {db.nq}); expansions p50/p90/p99/max {pct}")
         x = torch.zeros(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
         y = torch.empty_like(x)
         y.copy_(x)  # reads 1 GiB, writes 1 GiB

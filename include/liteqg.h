@@ -348,6 +348,13 @@ lqg_status lqg_h2_probe(lqg_index* index, const float* queries, uint64_t nq,
  * (third/ngt/hashset.hpp:30-110). */
 lqg_status lqg_h2_set_overflow_bits(lqg_index* index, uint32_t bits);
 
+/* FastScan search kernel choice.  Indexes of degree 32 with D = 64, 96 or 128
+ * run a kernel specialised for that record shape (offsets and the exact
+ * distance's length fixed at compile time); generic = 1 runs the general
+ * kernel for them too (same results; for comparisons).  Default 0.  No
+ * counterpart in the reference. */
+lqg_status lqg_h2_set_generic(lqg_index* index, int generic);
+
 /* Read-only views of the loaded index (host copies). */
 lqg_status lqg_info(const lqg_index* index, uint64_t* num_elements, uint64_t* dimension,
                     uint64_t* degree_bound, uint32_t* entry_point, uint32_t* ef);

@@ -84,6 +84,7 @@ _SIGS = {
     "lqg_h2_rotate": [_P, _P, C.c_uint64, C.c_int, _P],
     "lqg_h2_probe": [_P, _P, C.c_uint64, _P, _P, _P, _P, _P, _P],
     "lqg_h2_set_overflow_bits": [_P, C.c_uint32],
+    "lqg_h2_set_generic": [_P, C.c_int],
     "lqg_graph_create": [_P, _P, C.c_uint64, _P, _P, C.c_uint32, C.c_uint32, C.c_int, _P, _P, _P,
                          C.POINTER(_P)],
     "lqg_graph_launch": [_P, _P],

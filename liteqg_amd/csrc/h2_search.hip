@@ -3,8 +3,9 @@
 // build (quantization/rabitq.hpp:23-108), the query rotation by the FHT or
 // one FP32 GEMM on the matrix cores, and record row copies.  The search and
 // overflow kernels themselves are in h2_kernel.h, instantiated by
-// h2_kern_*.hip.
-#include "h2_kernel.h"
+// h2_kern_*.hip (general) and h2_fast.h, instantiated by h2_fast_n*.hip
+// (specialised for D = 64, 96, 128 at degree 32).
+#include "h2_fast.h"
 
 namespace lqg {
 
@@ -13,6 +14,9 @@ using SearchFn = void (*)(H2Dev, H2Job, uint32_t, Layout);
 SearchFn h2_select_ks1(int ef_cap, int cw, int occ);
 SearchFn h2_select_ks2(int ef_cap, int cw, int occ);
 SearchFn h2_select_overflow(int ef_cap, int cw, int ks);
+SearchFn h2_select_fast_n4(int ef_cap, int ks, int occ);
+SearchFn h2_select_fast_n6(int ef_cap, int ks, int occ);
+SearchFn h2_select_fast_n8(int ef_cap, int ks, int occ);
 
 namespace {
 
@@ -272,12 +276,21 @@ int code_words(const H2Dev& ix) {
 // Result-buffer register slots for k (0: k above 128, not supported).
 int rb_slots(uint32_t kmax) { return kmax <= 64 ? 1 : kmax <= 128 ? 2 : 0; }
 
-SearchFn select_search(int ef_cap, int cw, int occ, uint32_t kmax) {
-    switch (rb_slots(kmax)) {
-        case 1: return h2_select_ks1(ef_cap, cw, occ);
-        case 2: return h2_select_ks2(ef_cap, cw, occ);
-        default: return nullptr;
+// The search kernel: the one specialised for the index's record shape where
+// there is one (unless ix.generic), else the general one.
+SearchFn select_search(const H2Dev& ix, int ef_cap, int occ, uint32_t kmax) {
+    const int ks = rb_slots(kmax);
+    if (ks == 0) return nullptr;
+    if (!ix.generic) {
+        switch (h2_fast_nj(ix)) {
+            case 4: return h2_select_fast_n4(ef_cap, ks, occ);
+            case 6: return h2_select_fast_n6(ef_cap, ks, occ);
+            case 8: return h2_select_fast_n8(ef_cap, ks, occ);
+            default: break;
+        }
     }
+    const int cw = code_words(ix);
+    return ks == 1 ? h2_select_ks1(ef_cap, cw, occ) : h2_select_ks2(ef_cap, cw, occ);
 }
 
 // The search's LDS layout: the query stays in LDS only for dims the
@@ -307,7 +320,7 @@ bool h2_occupancy_ok(const H2Dev& ix) { return code_words(ix) <= 4; }
 bool h2_kmax_ok(uint32_t kmax) { return rb_slots(kmax) != 0; }
 
 bool h2_variant_ok(const H2Dev& ix, int occupancy, uint32_t kmax) {
-    return select_search(128, code_words(ix), occupancy, kmax) != nullptr;
+    return select_search(ix, 128, occupancy, kmax) != nullptr;
 }
 
 size_t h2_lds_bytes(const H2Dev& ix, const H2Cfg& cfg, uint32_t kmax) {
@@ -319,7 +332,7 @@ int h2_groups(const H2Dev& ix, const H2Cfg& cfg, uint32_t kmax, uint32_t nq) {
     const int need = static_cast<int>((nq + cfg.waves_per_group - 1) / cfg.waves_per_group);
     int groups = cfg.groups;
     if (groups <= 0) {
-        SearchFn k = select_search(cfg.ef_cap, code_words(ix), cfg.occupancy, kmax);
+        SearchFn k = select_search(ix, cfg.ef_cap, cfg.occupancy, kmax);
         int dev = 0, cus = 0, per_cu = 0;
         if (!k || hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -337,7 +350,7 @@ int h2_groups(const H2Dev& ix, const H2Cfg& cfg, uint32_t kmax, uint32_t nq) {
 
 hipError_t h2_launch_search(const H2Dev& ix, const H2Job& job, const H2Cfg& cfg,
                             hipStream_t stream) {
-    SearchFn k = select_search(cfg.ef_cap, code_words(ix), cfg.occupancy, job.kstride);
+    SearchFn k = select_search(ix, cfg.ef_cap, cfg.occupancy, job.kstride);
     if (!k || cfg.waves_per_group < 1 || cfg.waves_per_group > 4 || cfg.hash_bits < 6 ||
         cfg.hash_bits > 14 || job.kstride == 0)
         return hipErrorInvalidValue;

@@ -522,6 +522,13 @@ lqg_status lqg_h2_set_overflow_bits(lqg_index* h, uint32_t bits) {
     return LQG_OK;
 }
 
+lqg_status lqg_h2_set_generic(lqg_index* h, int generic) {
+    if (!h) return fail(LQG_ERR_ARG, "null index");
+    if (generic != 0 && generic != 1) return fail(LQG_ERR_ARG, "generic must be 0 or 1");
+    h->h2_generic = generic != 0;
+    return LQG_OK;
+}
+
 lqg_status lqg_h2_set_rotation(lqg_index* h, lqg_rotation method) {
     if (!h) return fail(LQG_ERR_ARG, "null index");
     if (method != LQG_ROTATION_FHT && method != LQG_ROTATION_GEMM)

@@ -144,6 +144,7 @@ struct lqg_index {
     lqg::H2Cfg h2_launch[lqg::kClasses];
     bool h2_launch_set[lqg::kClasses] = {};
     lqg_rotation h2_rotation = LQG_ROTATION_FHT;
+    bool h2_generic = false;     // lqg_h2_set_generic
     uint32_t h2_gvis_bits = 14;  // per-wave overflow visited table, log2 entries (0: none)
 
     lqg::H2Dev h2dev() const {
@@ -159,6 +160,7 @@ struct lqg_index {
         ix.off_codes = h2_off_codes;
         ix.off_fac = h2_off_fac;
         ix.off_ids = h2_off_ids;
+        ix.generic = h2_generic ? 1u : 0u;
         return ix;
     }
 

@@ -83,6 +83,7 @@ struct H2Dev {
     const float* signs;     // p rotation signs, +-1/sqrt(p)
     uint32_t n, dim, deg, p, ep;
     uint32_t stride, off_codes, off_fac, off_ids;  // bytes
+    uint32_t generic = 0;  // 1: the generic search kernel where a specialised one exists
 };
 
 // One batch of queries for the FastScan search kernel.

@@ -180,6 +180,12 @@ class Index:
         query fails instead (diagnostic)."""
         check(lib().lqg_h2_set_overflow_bits(self._h, int(bits)))
 
+    def set_generic_kernel(self, generic: bool) -> None:
+        """Run the general FastScan kernel even where one specialised for the
+        index's record shape exists (D = 64, 96, 128 at degree 32); same
+        results (for comparisons)."""
+        check(lib().lqg_h2_set_generic(self._h, 1 if generic else 0))
+
     @property
     def padded_dim(self) -> int:
         p = 1
