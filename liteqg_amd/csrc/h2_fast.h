@@ -67,8 +67,76 @@ constexpr uint32_t kChecked = 0x80000000u;
 // SGPR operand per VALU instruction); the s_nop covers a lane select that a
 // VALU instruction wrote.
 __device__ __forceinline__ uint32_t write_lane(uint32_t old, uint32_t v, uint32_t l) {
-    asm("s_nop 3\n\tv_writelane_b32 %0, %1, m0" : "+v"(old) : "s"(v), "{m0}"(l));
+    asm("s_nop 3\n\tv_writelane_b32 %0, %1, m0" : "+v"(old) : "s"(uniform(v)), "{m0}"(uniform(l)));
     return old;
+}
+
+// m with bit b cleared (one scalar instruction).
+__device__ __forceinline__ uint64_t clear_bit(uint64_t m, uint32_t b) {
+    m = (static_cast<uint64_t>(uniform(static_cast<uint32_t>(m >> 32))) << 32) |
+        uniform(static_cast<uint32_t>(m));
+    asm("s_bitset0_b64 %0, %1" : "+s"(m) : "s"(uniform(b)));
+    return m;
+}
+
+// Pops the first unchecked entry of the pool (slot s on): its id, marked
+// checked in I and cleared from U; kEmpty if there is none.
+template <int S, int s = 0>
+__device__ __forceinline__ uint32_t pop_first(uint64_t (&U)[S], uint32_t (&I)[S]) {
+    if constexpr (s == S) {
+        return kEmpty;
+    } else {
+        if (U[s]) {
+            const uint32_t pl = static_cast<uint32_t>(__builtin_ctzll(U[s]));
+            U[s] = clear_bit(U[s], pl);
+            const uint32_t cur = readlane32(I[s], static_cast<int>(pl));
+            I[s] = write_lane(I[s], cur | kChecked, pl);
+            return cur;
+        }
+        return pop_first<S, s + 1>(U, I);
+    }
+}
+
+// scan_sum with the eight lookups of a code word issued before their sums.
+template <int CW>
+__device__ __forceinline__ uint32_t scan_sum_grouped(const uint32_t (&cw)[CW], LutBase lb) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < CW; ++k) {
+        const uint32_t t = cw[k] & 0x0F0F0F0Fu, v = cw[k] & 0xF0F0F0F0u;
+        uint32_t e[4], o[4];
+#define LQG_SCAN_BYTE2(J)                                                                   \
+    {                                                                                       \
+        const uint32_t b = 4u * static_cast<uint32_t>(k) + (J);                             \
+        e[J] = reinterpret_cast<lds_u8*>(static_cast<uintptr_t>(add_byte<J>(lb.even, t)))[16u * b]; \
+        o[J] = reinterpret_cast<lds_u8*>(static_cast<uintptr_t>(add_byte<J>(lb.odd, v)))   \
+            [256u * (b >> 4) + (b & 15u)];                                                  \
+    }
+        LQG_SCAN_BYTE2(0)
+        LQG_SCAN_BYTE2(1)
+        LQG_SCAN_BYTE2(2)
+        LQG_SCAN_BYTE2(3)
+#undef LQG_SCAN_BYTE2
+        __builtin_amdgcn_sched_barrier(0);
+        s += (e[0] + o[0]) + (e[1] + o[1]) + (e[2] + o[2]) + (e[3] + o[3]);
+    }
+    return s;
+}
+
+// lane_estimate for two lanes per neighbour, the pair sum one DPP add.
+template <int CW>
+__device__ __forceinline__ float estimate2(const uint32_t (&cw)[CW], LutBase lb, const Prep& pr,
+                                           float sqr_y, float tx, float dq, float vq) {
+    uint32_t s = scan_sum_grouped<CW>(cw, lb);
+    // s + the other lane's s (quad_perm [1, 0, 3, 2]); the s_nop covers the
+    // VALU write of s before a DPP read
+    asm("s_nop 1\n\tv_add_u32_dpp %0, %0, %0 quad_perm:[1,0,3,2] row_mask:0xf bank_mask:0xf"
+        : "+v"(s));
+    const int32_t r = static_cast<int32_t>(static_cast<int16_t>(static_cast<uint16_t>(s & 0xFFFFu)));
+    const float f = static_cast<float>(r * 2 - pr.sumq);
+    const float b = fmaf(vq, pr.lo, tx + sqr_y);
+    const float c = (f * dq) * pr.width;
+    return c + b;
 }
 
 // visited_.get for the lanes in `active` (the LDS table): the lanes whose id
@@ -239,31 +307,12 @@ __device__ bool h2_fast_one(const H2Dev& ix, const H2Job& job, uint32_t q, const
 
     for (;;) {
         // pop: first unchecked entry (buffer.hpp:52-60)
-        int fs = S;
-#pragma unroll
-        for (int s = S - 1; s >= 0; --s)
-            if (U[s]) fs = s;
-        if (fs == S) break;
-        uint32_t cur = 0;
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            if (s == fs) {
-                const uint32_t pl = static_cast<uint32_t>(__builtin_ctzll(U[s]));
-                U[s] &= U[s] - 1;
-                cur = readlane32(I[s], static_cast<int>(pl));
-                I[s] = write_lane(I[s], cur | kChecked, pl);
-            }
-        }
+        const uint32_t cur = pop_first<S>(U, I);
+        if (cur == kEmpty) break;  // ids are below 2^31
         ++n_pop;
-        const uint32_t vs = v_test_set_fast(vis, cur, lane);
-        if (vs != kVisNew) {
-            if (vs == kVisSeen) continue;
-            full = true;
-            break;
-        }
-        ++n_exp;
-
-        // the node's record: raw row partials, codes, factors, ids
+        // the node's record (raw row partials, codes, factors, ids), asked for
+        // before the visited test so that the two latencies overlap; a node
+        // already expanded (about one pop in eight) leaves it unused
         const unsigned char* rec = ix.rec + static_cast<size_t>(cur) * Sh::kStride;
         const gptr<float> rf = uniform_gptr<float>(rec);
         const gptr<uint32_t> ru = uniform_gptr<uint32_t>(rec);
@@ -275,9 +324,16 @@ __device__ bool h2_fast_one(const H2Dev& ix, const H2Job& job, uint32_t q, const
         const float tx = rf[Sh::kOffFac / 4u + lv];
         const float dq = rf[Sh::kOffFac / 4u + Sh::kDeg + lv];
         const float vq = rf[Sh::kOffFac / 4u + 2u * Sh::kDeg + lv];
+        const uint32_t vs = v_test_set_fast(vis, cur, lane);
+        if (vs != kVisNew) {
+            if (vs == kVisSeen) continue;
+            full = true;
+            break;
+        }
+        ++n_exp;
         // scan_neighbors (qg.hpp:355): exact distance to the node, estimates
         const float sqr_y = exact_fast<NJ>(xr, qr);
-        const float est = lane_estimate<CW>(cw, lutb, 2u, pr, sqr_y, tx, dq, vq);
+        const float est = estimate2<CW>(cw, lutb, pr, sqr_y, tx, dq, vq);
         // sb_key(est, seq + lv)
         const uint32_t klo = ((0x7FFFFFFFu - seq) << 1) - lv2;
         const uint64_t key = (static_cast<uint64_t>(sortable(est)) << 32) | klo;
@@ -293,19 +349,28 @@ __device__ bool h2_fast_one(const H2Dev& ix, const H2Job& job, uint32_t q, const
 #pragma unroll
             for (int s = 0; s < S; ++s) nlt[s] = 0;
             uint32_t fp = 0xFFFFu;  // lanes not admitted keep a position past the pool
+            // two neighbours per step (independent chains); a step with one
+            // left takes key 0 for the second, which is above nothing
             uint64_t mm = cm;
             while (mm) {
-                const uint32_t j = static_cast<uint32_t>(__builtin_ctzll(mm));
-                mm &= mm - 1;
-                const uint64_t kj = readlane64(key, static_cast<int>(j));
-                uint32_t below = static_cast<uint32_t>(__popcll(__ballot(key < kj) & cm));
+                const uint32_t j1 = static_cast<uint32_t>(__builtin_ctzll(mm));
+                mm = clear_bit(mm, j1);
+                const bool two = mm != 0;
+                const uint32_t j2 = two ? static_cast<uint32_t>(__builtin_ctzll(mm)) : j1;
+                mm = clear_bit(mm, j2);
+                const uint64_t k1 = readlane64(key, static_cast<int>(j1));
+                const uint64_t k2 = two ? readlane64(key, static_cast<int>(j2)) : 0ull;
+                uint32_t b1 = static_cast<uint32_t>(__popcll(__ballot(key < k1) & cm));
+                uint32_t b2 = static_cast<uint32_t>(__popcll(__ballot(key < k2) & cm));
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    const bool lt = P[s] < kj;
-                    below += static_cast<uint32_t>(__popcll(__ballot(lt)));
-                    nlt[s] += lt ? 1u : 0u;
+                    const bool l1 = P[s] < k1, l2 = P[s] < k2;
+                    b1 += static_cast<uint32_t>(__popcll(__ballot(l1)));
+                    b2 += static_cast<uint32_t>(__popcll(__ballot(l2)));
+                    nlt[s] += (l1 ? 1u : 0u) + (l2 ? 1u : 0u);
                 }
-                fp = write_lane(fp, below, j);
+                fp = write_lane(fp, b2, j2);  // j2 == j1 with one left: rewritten next
+                fp = write_lane(fp, b1, j1);
             }
             const uint32_t nsize = min(size + m3, ef);
             // merge through LDS: every entry that stays goes to its new position
