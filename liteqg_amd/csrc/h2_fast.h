@@ -360,14 +360,26 @@ __device__ bool h2_fast_one(const H2Dev& ix, const H2Job& job, uint32_t q, const
                 mm = clear_bit(mm, j2);
                 const uint64_t k1 = readlane64(key, static_cast<int>(j1));
                 const uint64_t k2 = two ? readlane64(key, static_cast<int>(j2)) : 0ull;
-                uint32_t b1 = static_cast<uint32_t>(__popcll(__ballot(key < k1) & cm));
-                uint32_t b2 = static_cast<uint32_t>(__popcll(__ballot(key < k2) & cm));
+                // every compare first, each into its own lane mask, then the
+                // counts (no compare waits for the scalar unit)
+                const uint64_t r1 = __ballot(key < k1), r2 = __ballot(key < k2);
+                bool l1[S], l2[S];
+                uint64_t a1[S], a2[S];
 #pragma unroll
                 for (int s = 0; s < S; ++s) {
-                    const bool l1 = P[s] < k1, l2 = P[s] < k2;
-                    b1 += static_cast<uint32_t>(__popcll(__ballot(l1)));
-                    b2 += static_cast<uint32_t>(__popcll(__ballot(l2)));
-                    nlt[s] += (l1 ? 1u : 0u) + (l2 ? 1u : 0u);
+                    l1[s] = P[s] < k1;
+                    l2[s] = P[s] < k2;
+                    a1[s] = __ballot(l1[s]);
+                    a2[s] = __ballot(l2[s]);
+                }
+#pragma unroll
+                for (int s = 0; s < S; ++s) nlt[s] += (l1[s] ? 1u : 0u) + (l2[s] ? 1u : 0u);
+                uint32_t b1 = static_cast<uint32_t>(__popcll(r1 & cm));
+                uint32_t b2 = static_cast<uint32_t>(__popcll(r2 & cm));
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    b1 += static_cast<uint32_t>(__popcll(a1[s]));
+                    b2 += static_cast<uint32_t>(__popcll(a2[s]));
                 }
                 fp = write_lane(fp, b2, j2);  // j2 == j1 with one left: rewritten next
                 fp = write_lane(fp, b1, j1);
