@@ -61,7 +61,7 @@ SERVE_MIX_RATES = (5_000.0, 20_000.0, 50_000.0)
 SERVE_MIX_LANES, SERVE_MIX_BATCH = 16, 2048
 TARGET_RECALL = 0.95
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "h1_search_kernel_c2_traffic.md")
-TRAFFIC_FILE_FS = os.path.join(ROOT, "profiles", "h2_search_kernel_c2_traffic.md")
+TRAFFIC_FILE_FS = os.path.join(ROOT, "profiles", "h2_fast_kernel_c2_traffic.md")
 
 
 def log(*a):

@@ -12,6 +12,9 @@ Fixtures written (all small):
   <name>.npz             base/queries shapes, queries, and the reference's
                          search() ids for each ef in EFS at k=10
   funcs_d<D>.npz         sq8_dequantize + l2_sqr vectors for several D
+  h2_<name>.npz          the reference's search_qg on the FastScan rows:
+                         sidecar, query preparation, ids at H2_EFS, k=10
+  h2u_<name>.npz         search_qg ids at the H2_UPDATE_CASES (ef, k) pairs
 
 Usage:  python tests/golden/make_golden.py
 """
@@ -33,6 +36,10 @@ from liteqg_amd import datasets  # noqa: E402
 DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
 DRIVER_H2 = os.path.join(ROOT, "oracle", "_ref", "ref_h2")
 H2_EFS = (10, 50, 100)
+# (ef, k) pairs whose searches mostly expand fewer than k nodes, so the
+# reference's update_results tail (qg.hpp:395-418) and its zero-filled id
+# snapshot (buffer.hpp:81-102) decide the output
+H2_UPDATE_CASES = ((1, 10), (2, 10), (4, 10), (16, 100), (64, 100))
 H2_INDEXES = ("g_uniform_n2000_d64", "g_intmix_n1200_d128", "g_unitmix_n1000_d96_deg64")
 EFS = (10, 20, 50, 100)
 K = 10
@@ -124,6 +131,29 @@ def make_h2(name, nq=100, nprep=8, seed=3):
                         efs=np.array(H2_EFS, np.int64), k=np.int64(K), **out)
 
 
+def make_h2_update(name, nq=100, seed=3):
+    """search_qg at small ef / large k (update_results), same signs and
+    queries as make_h2."""
+    z = np.load(os.path.join(HERE, name + ".npz"))
+    n, d, deg = (int(x) for x in z["shape"])
+    h = np.load(os.path.join(HERE, "h2_" + name + ".npz"))
+    out = {}
+    with tempfile.TemporaryDirectory() as tmp:
+        sp, qp = os.path.join(tmp, "s.f32"), os.path.join(tmp, "q.f32")
+        h["signs"].tofile(sp)
+        q = np.ascontiguousarray(h["queries"][:nq])
+        q.tofile(qp)
+        ipath = os.path.join(HERE, name + ".index")
+        for ef, k in H2_UPDATE_CASES:
+            o, sc, pr = (os.path.join(tmp, x) for x in ("o.u32", "side.bin", "prep.bin"))
+            subprocess.run([DRIVER_H2, "search", ipath, str(n), str(d), str(deg), sp, qp,
+                            str(nq), str(ef), str(k), o, sc, pr, "0"], check=True,
+                           stdout=subprocess.DEVNULL)
+            out[f"ids_ef{ef}_k{k}"] = np.fromfile(o, np.uint32).reshape(nq, k)
+    np.savez_compressed(os.path.join(HERE, "h2u_" + name + ".npz"), queries=q,
+                        cases=np.array(H2_UPDATE_CASES, np.int64), **out)
+
+
 def make_buffers(cap=16, nops=4000, seed=9):
     """SearchBuffer / ResultBuffer operation traces (utils/buffer.hpp)."""
     rng = np.random.default_rng(seed)
@@ -171,10 +201,14 @@ def main():
         for name, spec in INDEXES.items():
             make_index(name, *spec)
             print("wrote", name)
-    make_buffers()
+    if "--h2u-only" not in sys.argv:
+        make_buffers()
+        for name in H2_INDEXES:
+            make_h2(name)
+            print("wrote h2", name)
     for name in H2_INDEXES:
-        make_h2(name)
-        print("wrote h2", name)
+        make_h2_update(name)
+        print("wrote h2u", name)
 
 
 if __name__ == "__main__":

@@ -90,6 +90,39 @@ def test_search_matches_reference(name):
 
 
 @pytest.mark.parametrize("name", H2_NAMES)
+def test_update_results_matches_reference(oracle_mod, name):
+    """Searches that expand fewer than k nodes (ef 1-4 at k 10, ef 16/64 at
+    k 100): the update_results tail (qg.hpp:395-418), including the probe of
+    node 0 through the zero-filled id snapshot (buffer.hpp:81-102).  Ids equal
+    the reference's search_qg (h2u_*.npz) and ids and distance bits equal
+    the oracle's, in one launch per case and in one launch holding every
+    case."""
+    arr, _ = load_golden(name)
+    h = np.load(f"{GOLDEN}/h2_{name}.npz")
+    u = np.load(f"{GOLDEN}/h2u_{name}.npz")
+    ix = _index_from_golden(name)
+    ix.set_sidecar(h["signs"], h["codes"], h["factors"], layout="blocks")
+    ix.set_mode("fastscan")
+    H = oracle_mod.H2Arrays(arr, h["signs"], h["codes"], h["factors"], 0)
+    q = u["queries"]
+    efs, ks = [], []
+    for ef, k in u["cases"]:
+        ids, dis = ix.search_batch(q, int(k), ef=int(ef), return_distances=True)
+        assert np.array_equal(ids, u[f"ids_ef{ef}_k{k}"]), f"ef={ef} k={k}"
+        e_ids, e_dis, _ = oracle_mod.search_h2_batch(H, q, int(ef), int(k))
+        assert np.array_equal(ids, e_ids)
+        assert np.array_equal(_bits(dis), _bits(e_dis))
+        efs.append(np.full(q.shape[0], ef, np.uint32))
+        ks.append(np.full(q.shape[0], k, np.uint32))
+    # every case in one launch: per-query ef and k
+    Q = np.concatenate([q] * len(u["cases"]))
+    ids = ix.search_batch(Q, np.concatenate(ks), ef=np.concatenate(efs))
+    for c, (ef, k) in enumerate(u["cases"]):
+        got = ids[c * q.shape[0]:(c + 1) * q.shape[0], :k]
+        assert np.array_equal(got, u[f"ids_ef{ef}_k{k}"]), f"mixed launch ef={ef} k={k}"
+
+
+@pytest.mark.parametrize("name", H2_NAMES)
 def test_device_sidecar_matches_oracle_and_reference(oracle_mod, name):
     """The device-built sidecar equals the oracle's; searching with it gives
     the reference's ids on the fixtures."""

@@ -141,6 +141,56 @@ def test_served_results_equal_alone_and_in_mix(tmp_path, fastscan):
     assert np.all(lat > 0)
 
 
+@pytest.mark.parametrize("fastscan", [False, True], ids=["sq8", "fastscan"])
+def test_served_768d_results_equal_alone_and_in_mix(oracle_mod, tmp_path, fastscan):
+    """The 768-d serving shape: unit-norm mixture, degree 32, per-request ef in
+    64..512 and k in {10, 100}.  Each served result equals the batch search of
+    the same query (and, for a sample, the oracle's), alone and in the mix."""
+    from liteqg_amd import datasets
+    from liteqg_amd.serving import Server, poisson_arrivals
+    from liteqg_amd.symphonyqg import Index
+
+    O = oracle_mod
+    n, d, nq = 3000, 768, 1500
+    base, Q = datasets.make("unit_mix", n, d, nq, seed=2, centres=16, sigma=0.25,
+                            centre_law="normal")
+    ix = Index("QG", "L2", n, d, 32)
+    ix.build_index(base, 200, num_iter=2)
+    p = tmp_path / "s768.index"
+    ix.save(str(p))
+    arr = O.read_index(str(p), n, d, 32)
+    H = None
+    if fastscan:
+        ix.build_sidecar(seed=6)
+        ix.set_mode("fastscan")
+        s, codes, fac = ix.get_sidecar()
+        H = O.H2Arrays(arr, s, codes, fac, 1)
+    rng = np.random.default_rng(8)
+    ef = rng.integers(64, 513, nq).astype(np.uint32)
+    k = rng.choice([10, 100], nq).astype(np.uint32)
+    want, want_d = ix.search_batch(Q, k, ef=ef, order="nearest", return_distances=True)
+    for i in range(0, nq, 50):
+        ki = int(k[i])
+        if H is None:
+            e_ids, e_d, _ = O.search_canon_batch(arr, Q[i:i + 1], int(ef[i]), ki, "nearest")
+        else:
+            e_ids, e_d, _ = O.search_h2_batch(H, Q[i:i + 1], int(ef[i]), ki)
+        assert np.array_equal(want[i, :ki], e_ids[0]), i
+        assert np.array_equal(want_d[i, :ki].view(np.uint32), e_d[0].view(np.uint32)), i
+    with Server(ix, lanes=4, max_batch=256, slots=2048, ef_max=512, kmax=100) as srv:
+        for i in range(20):
+            t = srv.submit(Q[i], int(ef[i]), int(k[i]))
+            ids, dis = srv.wait(t, int(k[i]))
+            assert np.array_equal(ids, want[i, :k[i]]), i
+            assert np.array_equal(dis.view(np.uint32), want_d[i, :k[i]].view(np.uint32)), i
+        lat, ids = srv.replay(Q, ef, k, poisson_arrivals(nq, 1.0e6, seed=3))
+        for i in range(nq):
+            assert np.array_equal(ids[i, :k[i]], want[i, :k[i]]), i
+        st = srv.stats()
+    assert st["completed"] == st["submitted"] == nq + 20
+    assert st["max_batch_seen"] > 1
+
+
 def test_server_admission_and_errors(tmp_path):
     from liteqg_amd._lib import LqgError
     from liteqg_amd.serving import Server

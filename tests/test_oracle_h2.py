@@ -197,6 +197,25 @@ def test_search_qg_matches_reference(oracle_mod, name):
 
 
 @pytest.mark.parametrize("name", H2_NAMES)
+def test_update_results_matches_reference(oracle_mod, name):
+    """update_results (qg.hpp:395-418): at ef 1-4 with k 10 and ef 16/64 with
+    k 100 most searches expand fewer than k nodes, so the result list is
+    filled from the neighbours of the id snapshot, whose unfilled slots are
+    zero (buffer.hpp:81-102) and probe node 0.  The reference's search_qg ids
+    (h2u_*.npz), 100 queries per case."""
+    ix, h = _h2(name)
+    u = np.load(f"{GOLDEN}/h2u_{name}.npz")
+    arr = oracle_mod.H2Arrays(ix, h["signs"], h["codes"], h["factors"], 0)
+    short = 0
+    for ef, k in u["cases"]:
+        ids, dist, st = oracle_mod.search_h2_batch(arr, u["queries"], int(ef), int(k))
+        assert np.array_equal(ids, u[f"ids_ef{ef}_k{k}"]), f"ef={ef} k={k}"
+        short += int((st[:, 0] < k).sum())
+        assert np.all(np.diff(dist, axis=1) >= 0)
+    assert short >= 400  # the tail runs in most of the 500 searches
+
+
+@pytest.mark.parametrize("name", H2_NAMES)
 def test_sidecar_codes_equal_reference(oracle_mod, name):
     """rabitq_codes (quantization/rabitq.hpp:23-59): this build's sidecar has
     the reference's sign codes bit for bit (in vector-major layout)."""
