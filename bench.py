@@ -110,7 +110,7 @@ def recall_at_k(found, gt, k):
 class DeviceBatch:
     """Device-resident queries/results and the raw C ABI call on torch's stream."""
 
-    def __init__(self, ix, Q, k):
+    def __init__(self, ix, Q, k, ef_per_query=None, k_per_query=None):
         import torch
 
         from liteqg_amd import _lib
@@ -121,6 +121,12 @@ class DeviceBatch:
         self.nq = Q.shape[0]
         self.k = k
         self.q = torch.from_numpy(Q).cuda()
+        # optional per-query ef and k (then launch's ef is their maximum and
+        # k the largest k)
+        self.d_ef = (None if ef_per_query is None else
+                     torch.from_numpy(np.ascontiguousarray(ef_per_query, np.int32)).cuda())
+        self.d_k = (None if k_per_query is None else
+                    torch.from_numpy(np.ascontiguousarray(k_per_query, np.int32)).cuda())
         self.ids = torch.empty((self.nq, k), dtype=torch.int32, device="cuda")
         self.stats = torch.empty((self.nq, 2), dtype=torch.int32, device="cuda")
         # a stream of our own (default priority): its handle is never NULL, so
@@ -130,7 +136,9 @@ class DeviceBatch:
 
     def launch(self, ef, with_stats=False):
         self.check(self.L.lqg_search_batch_device(
-            self.h, C.c_void_p(self.q.data_ptr()), self.nq, None, None, ef, self.k, 1,
+            self.h, C.c_void_p(self.q.data_ptr()), self.nq,
+            None if self.d_ef is None else C.c_void_p(self.d_ef.data_ptr()),
+            None if self.d_k is None else C.c_void_p(self.d_k.data_ptr()), ef, self.k, 1,
             C.c_void_p(self.ids.data_ptr()), None,
             C.c_void_p(self.stats.data_ptr()) if with_stats else None,
             C.c_void_p(self.stream.cuda_stream)))
@@ -308,6 +316,119 @@ def serving_mix_leg(args):
             log(f"serving mix {row}")
         out["engine"] = dict(srv.stats(), lanes=SERVE_MIX_LANES, max_batch=SERVE_MIX_BATCH,
                              slots=SERVE_SLOTS)
+    return out
+
+
+def wide_leg(args, host):
+    """The 960-d workload at full size: 1,000,000 x 960 non-negative mixture
+    (512 centres U[0,1]^960, sigma 0.05, clipped at 0), degree 64, graph built
+    on the device (build EF 500, 3 iterations); 1,000 queries from the same
+    process with per-query ef ~ U{64..1024} and k ~ {10, 100}, all of them in
+    one persistent launch (pool class of ef 1024, launch settings autotuned
+    for that class), in both search modes: SQ8 (order nearest) and FastScan
+    with its exact re-rank.  Recall@10 (first 10 ids of each row) against
+    FP64 brute force, QPS, and the roofline on algorithmic bytes:
+      SQ8:      4D + 8D + 4*deg*E + D*V + 4k  (E, V exact per query)
+      FastScan: 4D + E*(4D + deg*P/8 + 16*deg) + 4k
+    plus per-query parity against the oracle on a sample (ids and distance
+    bits)."""
+    import torch
+
+    from liteqg_amd import datasets
+    from liteqg_amd.symphonyqg import Index
+
+    n, d, deg, nq, efb = args.wide_n, 960, 64, 1000, 500
+    t = time.perf_counter()
+    base, Q = datasets.make("pos_mix", n, d, nq, seed=args.seed, centres=512, sigma=0.05)
+    ix = Index("QG", "L2", n, d, deg)
+    ix.build_index(base, efb, num_iter=3)
+    build_s = time.perf_counter() - t
+    log(f"wide: data + build {build_s:.1f}s")
+    gt = ground_truth(base, Q, 10)
+    rng = np.random.default_rng(args.seed + 4)
+    ef = rng.integers(64, 1025, nq).astype(np.uint32)
+    k = np.where(rng.random(nq) < 0.5, 10, 100).astype(np.uint32)
+    ef_max, kmax = int(ef.max()), int(k.max())
+    db = DeviceBatch(ix, Q, kmax, ef_per_query=ef, k_per_query=k)
+    p = ix.padded_dim
+    out = {"workload": f"pos_mix N={n} D={d} deg={deg} (512 centres U[0,1]^D, sigma 0.05, "
+                       f"clipped at 0), {nq} queries, ef ~ U{{64..1024}}, k ~ {{10, 100}}, "
+                       f"one launch; graph built on device with build EF {efb} x3 iterations",
+           "data_and_build_s": round(build_s, 1)}
+
+    def timed(mode):
+        tuned = ix.autotune(Q, ef_max, kmax)
+        db.launch(ef_max, with_stats=True)
+        torch.cuda.synchronize()
+        ids = db.results()
+        E, V = db.counters()
+        ms, _ = time_launches(db, ef_max, args.steps, args.warmup)
+        rec = recall_at_k(ids[:, :10], gt, 10)
+        return dict(mode=mode, launch=tuned, ms_per_step=round(ms, 4),
+                    qps=round(nq / ms * 1e3, 1), recall_at_10=round(rec, 4)), ids, E, V, ms
+
+    # SQ8: E and V of the reference's visited set from the exact counting
+    # launch, one call per distinct ef (the traversal does not depend on k)
+    row, ids_sq8, E, _, ms = timed("sq8")
+    ev = np.zeros((nq, 2), np.int64)
+    for e in np.unique(ef):
+        sel = np.nonzero(ef == e)[0]
+        ev[sel] = ix.count_visits(Q[sel], int(e), 10)
+    if not np.array_equal(ev[:, 0], E):
+        raise RuntimeError("wide: expansion counts differ between the timed and counting launch")
+    alg = float(np.sum(4 * d + 8 * d + 4 * deg * ev[:, 0] + d * ev[:, 1] + 4 * k.astype(np.int64)))
+    gbs = alg / (ms / 1e3) / 1e9
+    row["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                       "bytes_per_query": round(alg / nq, 1),
+                       "expansions_per_query": round(float(ev[:, 0].mean()), 2),
+                       "distinct_evaluations_per_query": round(float(ev[:, 1].mean()), 2)}
+    out["sq8"] = row
+    log(f"wide sq8: {row}")
+
+    ix.build_sidecar(seed=args.seed)
+    ix.set_mode("fastscan")
+    row, ids_fs, E, _, ms = timed("fastscan")
+    rec_bytes = 4 * d + deg * p // 8 + 16 * deg
+    alg = float(np.sum(4 * d + E.astype(np.int64) * rec_bytes + 4 * k.astype(np.int64)))
+    gbs = alg / (ms / 1e3) / 1e9
+    row["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                       "bytes_per_query": round(alg / nq, 1), "record_bytes": rec_bytes,
+                       "expansions_per_query": round(float(E.mean()), 2)}
+    out["fastscan"] = row
+    log(f"wide fastscan: {row}")
+
+    if not args.no_parity:
+        from oracle import oracle as O
+
+        sample = np.arange(0, nq, nq // args.wide_parity_nq)[:args.wide_parity_nq]
+        with tempfile.TemporaryDirectory() as tmp:
+            path = os.path.join(tmp, "w.index")
+            ix.save(path)
+            arr = O.read_index(path, n, d, deg)
+        s_, codes, fac = ix.get_sidecar()
+        H = O.H2Arrays(arr, s_, codes, fac, 1)
+        dis = {}
+        for mode in ("sq8", "fastscan"):
+            ix.set_mode(mode)
+            _, dis[mode] = ix.search_batch(Q, k, ef=ef, order="nearest", return_distances=True)
+        ok = {"sq8": True, "fastscan": True}
+        for i in sample:
+            ki, ei = int(k[i]), int(ef[i])
+            e_ids, e_d, _ = O.search_canon_batch(arr, Q[i:i + 1], ei, ki, "nearest")
+            ok["sq8"] &= bool(np.array_equal(ids_sq8[i, :ki], e_ids[0]) and np.array_equal(
+                dis["sq8"][i, :ki].view(np.uint32), e_d[0].view(np.uint32)))
+            e_ids, e_d, _ = O.search_h2_batch(H, Q[i:i + 1], ei, ki)
+            ok["fastscan"] &= bool(np.array_equal(ids_fs[i, :ki], e_ids[0]) and np.array_equal(
+                dis["fastscan"][i, :ki].view(np.uint32), e_d[0].view(np.uint32)))
+        out["parity_vs_oracle"] = {"queries": int(sample.size), "ids_and_dist_bits_equal": ok,
+                                   "ok": ok["sq8"] and ok["fastscan"]}
+        log(f"wide parity: {out['parity_vs_oracle']}")
+        del arr, H, codes, fac
+    ix.set_mode("sq8")
+    del ix, db, base
+    torch.cuda.empty_cache()
     return out
 
 
@@ -708,6 +829,12 @@ def main():
                     help="skip the full-size parity checks against the oracle and the reference")
     ap.add_argument("--parity-nq", type=int, default=2000,
                     help="queries of the full-size parity sample")
+    ap.add_argument("--no-wide", action="store_true",
+                    help="skip the 1M x 960 degree-64 workload (about 90 s)")
+    ap.add_argument("--wide-n", type=int, default=1_000_000,
+                    help="base vectors of the 960-d workload")
+    ap.add_argument("--wide-parity-nq", type=int, default=50,
+                    help="queries of the 960-d workload checked against the oracle")
     ap.add_argument("--serve-n", type=int, default=1_000_000,
                     help="base vectors of the 768-d serving workload")
     ap.add_argument("--serve-nq", type=int, default=10_000,
@@ -877,6 +1004,9 @@ def main():
     serving_mix = None if args.no_serve else serving_mix_leg(args)
 
     configs = None if args.no_configs else configs_leg(args, host)
+    wide = None if args.no_wide else wide_leg(args, host)
+    if wide:
+        configs = dict(configs or {}, posmix_1m_d960=wide)
 
     line = {
         "metric": "QPS at recall@10≥0.95 (batched vs plain GPU), 1M×128 deg-32; p99 ms",
@@ -938,7 +1068,8 @@ def main():
     if cpu:
         line["gpu_over_cpu"] = round(qps / cpu["value"], 1)
     print(json.dumps(line), flush=True)
-    bad = [name for name, p in (("SQ8", parity), ("FastScan", fastscan and fastscan["parity"]))
+    bad = [name for name, p in (("SQ8", parity), ("FastScan", fastscan and fastscan["parity"]),
+                                ("960-d", wide and wide.get("parity_vs_oracle")))
            if p and not p["ok"]]
     if bad:
         log(f"PARITY FAILED: {bad}")
