@@ -274,10 +274,11 @@ def serving_mix_leg(args):
     768-d vectors from a 2,048-centre mixture (centres N(0, I), sigma 0.25
     before normalisation), degree 32, graph built on the device (build EF 400, 3
     iterations); Poisson arrivals of single-query requests at 5k, 20k and
-    50k per second with per-request ef uniform in {64..512} and k in {10, 100}
-    (SQ8 path, order=nearest).  Reports p50/p99 latency and recall@10 (first
-    10 ids of each request against FP64 brute force) per rate, plus an
-    unthrottled run for the engine's ceiling."""
+    50k per second with per-request ef uniform in {64..512} and k in {10, 100},
+    served in both search modes: the SQ8 path (order=nearest) and the FastScan
+    path (1-bit estimates, exact re-rank).  Reports p50/p99 latency and
+    recall@10 (first 10 ids of each request against FP64 brute force) per
+    rate, plus an unthrottled run for the engine's ceiling."""
     from liteqg_amd import datasets
     from liteqg_amd.serving import Server, poisson_arrivals
     from liteqg_amd.symphonyqg import Index
@@ -299,23 +300,37 @@ def serving_mix_leg(args):
                        f"{nq} requests per rate, ef ~ U{{64..512}}, k ~ {{10, 100}}",
            "data_and_build_s": round(build_s, 1), "rates": []}
     log(f"serving mix: data + build {build_s:.1f}s")
-    with Server(ix, lanes=SERVE_MIX_LANES, max_batch=SERVE_MIX_BATCH, slots=SERVE_SLOTS,
-                ef_max=512, kmax=100) as srv:
-        srv.replay(Q[:1000], ef[:1000], k[:1000], np.zeros(1000, np.uint64))  # warm-up
-        for rate in SERVE_MIX_RATES + (None,):
-            arr = (np.zeros(nq, np.uint64) if rate is None
-                   else poisson_arrivals(nq, rate, seed=args.seed + 2))
-            lat, ids = srv.replay(Q, ef, k, arr)
-            span = float((arr.astype(np.float64) + lat).max() - float(arr[0])) / 1e9
-            row = {"rate_per_s": rate if rate is not None else "unthrottled",
-                   "p50_ms": round(float(np.percentile(lat, 50)) / 1e6, 4),
-                   "p99_ms": round(float(np.percentile(lat, 99)) / 1e6, 4),
-                   "achieved_qps": round(nq / span, 1),
-                   "recall_at_10": round(recall_at_k(ids[:, :10], gt, 10), 4)}
-            out["rates"].append(row)
-            log(f"serving mix {row}")
-        out["engine"] = dict(srv.stats(), lanes=SERVE_MIX_LANES, max_batch=SERVE_MIX_BATCH,
-                             slots=SERVE_SLOTS)
+
+    def serve(key):
+        rows = []
+        with Server(ix, lanes=SERVE_MIX_LANES, max_batch=SERVE_MIX_BATCH, slots=SERVE_SLOTS,
+                    ef_max=512, kmax=100) as srv:
+            srv.replay(Q[:1000], ef[:1000], k[:1000], np.zeros(1000, np.uint64))  # warm-up
+            for rate in SERVE_MIX_RATES + (None,):
+                arr = (np.zeros(nq, np.uint64) if rate is None
+                       else poisson_arrivals(nq, rate, seed=args.seed + 2))
+                lat, ids = srv.replay(Q, ef, k, arr)
+                span = float((arr.astype(np.float64) + lat).max() - float(arr[0])) / 1e9
+                row = {"rate_per_s": rate if rate is not None else "unthrottled",
+                       "p50_ms": round(float(np.percentile(lat, 50)) / 1e6, 4),
+                       "p99_ms": round(float(np.percentile(lat, 99)) / 1e6, 4),
+                       "achieved_qps": round(nq / span, 1),
+                       "recall_at_10": round(recall_at_k(ids[:, :10], gt, 10), 4)}
+                rows.append(row)
+                log(f"serving mix ({key}) {row}")
+            out[key + "_engine"] = dict(srv.stats(), lanes=SERVE_MIX_LANES,
+                                        max_batch=SERVE_MIX_BATCH, slots=SERVE_SLOTS)
+        return rows
+
+    # the SQ8 path (what the reference's search() runs), then the FastScan
+    # path with its exact re-rank on a device-built sidecar of the same graph
+    out["rates"] = serve("sq8")
+    t = time.perf_counter()
+    ix.build_sidecar(seed=args.seed)
+    ix.set_mode("fastscan")
+    out["fastscan_sidecar_build_s"] = round(time.perf_counter() - t, 2)
+    out["fastscan_rates"] = serve("fastscan")
+    ix.set_mode("sq8")
     return out
 
 
