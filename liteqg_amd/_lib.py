@@ -9,7 +9,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblqg.so")
+# LQG_LIB names another build of the same sources (the sanitizer build of
+# `python -m liteqg_amd.build --sanitize`, liteqg_amd/_san/liblqg.so)
+LIB_PATH = os.environ.get("LQG_LIB") or os.path.join(HERE, "liblqg.so")
 
 LQG_OK = 0
 ORDER = {"reference": 0, "nearest": 1}

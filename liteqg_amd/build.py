@@ -1,6 +1,12 @@
 """Compiles the gfx950 library liteqg_amd/liblqg.so with hipcc.
 
-    python -m liteqg_amd.build
+    python -m liteqg_amd.build              # liteqg_amd/liblqg.so
+    python -m liteqg_amd.build --sanitize   # liteqg_amd/_san/liblqg.so: host code
+                                            # with AddressSanitizer + UBSan
+
+The sanitizer build instruments host code only (GPU code is built as usual):
+load it with LQG_LIB=liteqg_amd/_san/liblqg.so and the clang ASan runtime
+preloaded (see oracle/Makefile, target check-san).
 
 All device code is built for gfx950 only, with packed-FP32 instruction
 selection disabled and FP contraction off (every fused multiply-add in the
@@ -49,14 +55,24 @@ def _compile(src: str, obj: str, extra, verbose: bool) -> None:
         raise RuntimeError(f"hipcc failed on {os.path.basename(src)}")
 
 
-def build(verbose: bool = False, extra=()) -> str:
+SAN_FLAGS = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+             "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-g", "-shared-libsan"]
+
+
+def build(verbose: bool = False, extra=(), sanitize: bool = False) -> str:
     """Compile every source into liblqg.so (always recompiles; one hipcc
-    process per source, in parallel, then one link)."""
+    process per source, in parallel, then one link).  sanitize: host code
+    with AddressSanitizer and UBSan, into liteqg_amd/_san/liblqg.so."""
     from concurrent.futures import ThreadPoolExecutor
 
     if shutil.which(HIPCC) is None and not os.path.exists(HIPCC):
         raise RuntimeError(f"hipcc not found at {HIPCC}")
-    objdir = os.path.join(HERE, "_obj")
+    lib_path = LIB
+    if sanitize:
+        extra = (*extra, *SAN_FLAGS)
+        lib_path = os.path.join(HERE, "_san", "liblqg.so")
+        os.makedirs(os.path.dirname(lib_path), exist_ok=True)
+    objdir = os.path.join(HERE, "_san_obj" if sanitize else "_obj")
     os.makedirs(objdir, exist_ok=True)
     srcs = sources()
     objs = [os.path.join(objdir, os.path.basename(s) + ".o") for s in srcs]
@@ -64,17 +80,18 @@ def build(verbose: bool = False, extra=()) -> str:
     with ThreadPoolExecutor(jobs) as ex:
         for f in [ex.submit(_compile, s, o, extra, verbose) for s, o in zip(srcs, objs)]:
             f.result()
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    tmp = lib_path + ".tmp"
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs,
+           *(SAN_FLAGS if sanitize else ())]
     if verbose:
         print(" ".join(cmd))
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc link failed")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib_path)
+    return lib_path
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, sanitize="--sanitize" in sys.argv[1:]))

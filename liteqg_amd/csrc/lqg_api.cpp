@@ -237,7 +237,10 @@ lqg_status check_queries(const float* q, uint64_t count) {
     return LQG_OK;
 }
 
-bool valid_order(lqg_order o) { return o == LQG_ORDER_REFERENCE || o == LQG_ORDER_NEAREST; }
+bool valid_order(const lqg_order& o) {
+    const int v = lqg::abi_enum(o);
+    return v == LQG_ORDER_REFERENCE || v == LQG_ORDER_NEAREST;
+}
 
 }  // namespace
 

@@ -303,11 +303,11 @@ extern "C" {
 
 lqg_status lqg_set_mode(lqg_index* h, lqg_mode mode) {
     if (!h) return fail(LQG_ERR_ARG, "null index");
-    if (mode == LQG_MODE_SQ8) {
-        h->mode = mode;
+    if (lqg::abi_enum(mode) == LQG_MODE_SQ8) {
+        h->mode = LQG_MODE_SQ8;
         return LQG_OK;
     }
-    if (mode != LQG_MODE_FASTSCAN) return fail(LQG_ERR_ARG, "unknown mode");
+    if (lqg::abi_enum(mode) != LQG_MODE_FASTSCAN) return fail(LQG_ERR_ARG, "unknown mode");
     if (!h->h2_ready)
         return fail(LQG_ERR_STATE, "FastScan mode needs a sidecar: build, set or load one first");
     h->mode = mode;
@@ -531,7 +531,7 @@ lqg_status lqg_h2_set_generic(lqg_index* h, int generic) {
 
 lqg_status lqg_h2_set_rotation(lqg_index* h, lqg_rotation method) {
     if (!h) return fail(LQG_ERR_ARG, "null index");
-    if (method != LQG_ROTATION_FHT && method != LQG_ROTATION_GEMM)
+    if (lqg::abi_enum(method) != LQG_ROTATION_FHT && lqg::abi_enum(method) != LQG_ROTATION_GEMM)
         return fail(LQG_ERR_ARG, "rotation must be LQG_ROTATION_FHT or LQG_ROTATION_GEMM");
     h->h2_rotation = method;
     return LQG_OK;
@@ -546,7 +546,7 @@ lqg_status lqg_h2_get_rotation(const lqg_index* h, lqg_rotation* method) {
 lqg_status lqg_h2_rotate(lqg_index* h, const float* queries, uint64_t nq, lqg_rotation method,
                          float* out) {
     if (!h || !queries || !out) return fail(LQG_ERR_ARG, "null argument");
-    if (method != LQG_ROTATION_FHT && method != LQG_ROTATION_GEMM)
+    if (lqg::abi_enum(method) != LQG_ROTATION_FHT && lqg::abi_enum(method) != LQG_ROTATION_GEMM)
         return fail(LQG_ERR_ARG, "rotation must be LQG_ROTATION_FHT or LQG_ROTATION_GEMM");
     if (nq == 0 || nq > 0xFFFFFFFFull) return fail(LQG_ERR_ARG, "nq must be in [1, 2^32)");
     if (!h->h2_ready) return fail(LQG_ERR_STATE, "no sidecar (rotation signs)");

@@ -3,8 +3,22 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 namespace lqg {
+
+// The bits of an enum argument that crossed the C ABI.  A C caller may pass
+// any int there, and reading a value outside the enumerators' range as the
+// enum type is undefined in C++, so arguments are validated on their bits
+// and converted back only once they are known to be valid.
+template <typename E>
+inline int abi_enum(const E& e) {
+    static_assert(sizeof(E) == sizeof(int), "C enums of this ABI are int-sized");
+    int v;
+    memcpy(&v, &e, sizeof v);
+    return v;
+}
+
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 constexpr int kWave = 64;

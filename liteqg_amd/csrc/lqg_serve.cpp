@@ -362,7 +362,8 @@ lqg_status lqg_serve_start(lqg_index* h, const lqg_serve_config* cfg, lqg_server
     if (cfg->kmax == 0) return fail(LQG_ERR_ARG, "kmax must be positive");
     if (cfg->ef_max == 0 || lqg::ef_class(cfg->ef_max) == 0)
         return fail(LQG_ERR_ARG, "ef_max must be in [1, 1024]");
-    if (cfg->order != LQG_ORDER_REFERENCE && cfg->order != LQG_ORDER_NEAREST)
+    if (lqg::abi_enum(cfg->order) != LQG_ORDER_REFERENCE &&
+        lqg::abi_enum(cfg->order) != LQG_ORDER_NEAREST)
         return fail(LQG_ERR_ARG, "unknown order");
     lqg_status st = lqg::ensure_device(h);
     if (st != LQG_OK) return st;

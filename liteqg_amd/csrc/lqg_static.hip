@@ -82,7 +82,7 @@ lqg_status lqg_search_batch_device_noalloc(lqg_index* h, const float* d_queries,
     if (!h || !d_queries || !d_results) return fail(LQG_ERR_ARG, "null argument");
     if (nq == 0 || nq > 0x7FFFFFFFull) return fail(LQG_ERR_ARG, "nq must be in [1, 2^31)");
     if (kmax == 0) return fail(LQG_ERR_ARG, "kmax must be positive");
-    if (order != LQG_ORDER_REFERENCE && order != LQG_ORDER_NEAREST)
+    if (lqg::abi_enum(order) != LQG_ORDER_REFERENCE && lqg::abi_enum(order) != LQG_ORDER_NEAREST)
         return fail(LQG_ERR_ARG, "unknown order");
     if (ef_max == 0) {
         if (d_ef) return fail(LQG_ERR_ARG, "ef_max must bound the per-query ef array");

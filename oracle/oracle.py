@@ -19,13 +19,17 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblqg_oracle.so")
+# LQG_ORACLE_LIB names another build of lqg_oracle.c (`make -C oracle san`:
+# oracle/_san/liblqg_oracle.so, built with AddressSanitizer and UBSan)
+LIB_PATH = os.environ.get("LQG_ORACLE_LIB") or os.path.join(HERE, "liblqg_oracle.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "ref_driver")
 FILL = 0xFFFFFFFF
 
 
 def build(force: bool = False) -> None:
     """Compile the C restatement (and oracle/_ref where /root/reference exists)."""
+    if os.environ.get("LQG_ORACLE_LIB"):
+        return
     if force or not os.path.exists(LIB_PATH):
         subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
 
