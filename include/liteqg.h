@@ -65,6 +65,18 @@ typedef enum {
     LQG_ROTATION_GEMM = 1
 } lqg_rotation;
 
+/* Where lqg_build_index takes each node's candidate neighbours from. */
+typedef enum {
+    /* A beam search of width EF on SQ8 distances from the entry point, its
+     * nearest min(EF, 750) entries (default). */
+    LQG_BUILD_SQ8 = 0,
+    /* The reference's find_candidates (qg.hpp:439-473, qg_builder.hpp:
+     * 202-231): a FastScan RaBitQ search of width EF from the entry point
+     * over the graph being built, whose codes and factors are recomputed
+     * for each iteration's graph, and every node it expands. */
+    LQG_BUILD_FASTSCAN = 1
+} lqg_build_candidates;
+
 typedef struct lqg_index lqg_index;
 
 /* Per-call counters (optional out-parameter of the batch entry points). */
@@ -110,6 +122,11 @@ lqg_status lqg_set_order(lqg_index* index, lqg_order order);
 lqg_status lqg_build_index(lqg_index* index, const float* data, uint64_t num_points,
                            uint64_t dimension, uint32_t ef_build, uint32_t num_iter,
                            uint32_t num_thread);
+
+/* Candidate source of later lqg_build_index calls (default LQG_BUILD_SQ8).
+ * The reference's builder always runs its FastScan search
+ * (qg_builder.hpp:202-231); this setter has no counterpart there. */
+lqg_status lqg_set_build_candidates(lqg_index* index, lqg_build_candidates source);
 
 /* Index.search(query, k) -- bindings.cpp:92-98, qg.hpp:235-244,298-344.
  * One query (D floats, host memory), one kernel launch; writes k ids to
@@ -337,6 +354,14 @@ lqg_status lqg_h2_rotate(lqg_index* index, const float* queries, uint64_t nq,
 lqg_status lqg_h2_probe(lqg_index* index, const float* queries, uint64_t nq,
                         const uint32_t* nodes, uint8_t* lut, float* lo_width, int32_t* sumq,
                         float* sqr_y, float* est);
+
+/* Diagnostic: the first cap nodes the FastScan search of each query (host
+ * memory, nq x D) expands at ef, in expansion order, to out (nq x cap,
+ * 0xFFFFFFFF past a query's last expansion) -- the candidate list the
+ * reference's builder takes from find_candidates (qg.hpp:439-473), and
+ * what lqg_build_index with LQG_BUILD_FASTSCAN takes per node. */
+lqg_status lqg_h2_expansions(lqg_index* index, const float* queries, uint64_t nq, uint32_t ef,
+                             uint32_t cap, uint32_t* out);
 
 /* FastScan visited set: each query's exact set lives in an LDS table sized by
  * its pool class and moves, past half full, to a per-wave table of 2^bits

@@ -17,6 +17,7 @@ LQG_OK = 0
 ORDER = {"reference": 0, "nearest": 1}
 MODE = {"sq8": 0, "fastscan": 1}
 ROTATION = {"fht": 0, "gemm": 1}
+BUILD_CANDIDATES = {"sq8": 0, "fastscan": 1}
 
 
 class LqgError(RuntimeError):
@@ -61,6 +62,7 @@ _SIGS = {
     "lqg_set_ef": [_P, C.c_uint32],
     "lqg_set_order": [_P, C.c_int],
     "lqg_build_index": [_P, _P, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32],
+    "lqg_set_build_candidates": [_P, C.c_int],
     "lqg_search": [_P, _P, C.c_uint32, _P],
     "lqg_search_batch": [_P, _P, C.c_uint64, _P, _P, C.c_uint32, C.c_int, _P, _P, _P],
     "lqg_search_batch_device": [_P, _P, C.c_uint64, _P, _P, C.c_uint32, C.c_uint32, C.c_int, _P,
@@ -85,6 +87,7 @@ _SIGS = {
     "lqg_h2_get_rotation": [_P, _P],
     "lqg_h2_rotate": [_P, _P, C.c_uint64, C.c_int, _P],
     "lqg_h2_probe": [_P, _P, C.c_uint64, _P, _P, _P, _P, _P, _P],
+    "lqg_h2_expansions": [_P, _P, C.c_uint64, C.c_uint32, C.c_uint32, _P],
     "lqg_h2_set_overflow_bits": [_P, C.c_uint32],
     "lqg_h2_set_generic": [_P, C.c_int],
     "lqg_graph_create": [_P, _P, C.c_uint64, _P, _P, C.c_uint32, C.c_uint32, C.c_int, _P, _P, _P,

@@ -19,6 +19,7 @@ namespace lqg {
 namespace {
 
 constexpr uint32_t kMaxCandidatePool = 750;  // qg_builder.hpp:30
+constexpr uint32_t kCandBuffer = 1024;       // candidates the prune kernels hold per node
 constexpr uint32_t kMaxPruned = 300;         // qg_builder.hpp:31
 constexpr uint64_t kSeed = 0x51A7E5EEDull;
 
@@ -164,15 +165,21 @@ lqg_status build_index(lqg_index* h, const float* data, uint32_t ef_build, uint3
     lqg_status st = ensure_device(h);
     if (st != LQG_OK) return st;
 
+    // Candidates of a node: the SQ8 beam's nearest min(EF, 750), or, with
+    // FastScan candidates, every node the FastScan search expands (up to the
+    // prune buffer's room beside the current neighbours), as find_candidates
+    // collects them (qg.hpp:439-473).
+    const bool fs_cand = h->build_candidates == LQG_BUILD_FASTSCAN;
     BuildParams p;
     p.n = static_cast<uint32_t>(n);
     p.dim = static_cast<uint32_t>(d);
     p.deg = static_cast<uint32_t>(g);
-    p.ncand = std::min(ef_build, kMaxCandidatePool);
+    p.ncand = fs_cand ? kCandBuffer - static_cast<uint32_t>(g) : std::min(ef_build, kMaxCandidatePool);
     p.pr_cap = kMaxPruned;
     p.seed = kSeed;
 
-    DevBuf<float> raw, kept_d, pr_d;
+    DevBuf<float> raw, kept_d, pr_d, rot;
+    DevBuf<uint32_t> out_k1;
     DevBuf<uint32_t> cand, kept_ids, kept_n, pr_ids, pr_n, in_cnt, in_off, in_ids, rev_k, rev_ko,
         rev_v, ctr;
     DevBuf<unsigned char> tmp;
@@ -222,11 +229,16 @@ lqg_status build_index(lqg_index* h, const float* data, uint32_t ef_build, uint3
     b.scan_tmp_bytes = tmp_bytes;
 
     lqg_status rs = LQG_OK;
+    if (fs_cand) rs = h2_build_begin(h, kSeed, rot);
     for (uint32_t it = 0; it < num_iter && rs == LQG_OK; ++it) {
         p.refine = it + 1 == num_iter;
         p.seed = kSeed + it;
-        rs = run_batch(h, raw.p, n, nullptr, nullptr, ef_build, p.ncand, LQG_ORDER_NEAREST, cand.p,
-                       nullptr, nullptr, h->stream);
+        if (fs_cand)
+            rs = h2_build_candidates(h, raw.p, rot.p, ef_build, cand.p, p.ncand, out_k1,
+                                     h->stream);
+        else
+            rs = run_batch(h, raw.p, n, nullptr, nullptr, ef_build, p.ncand, LQG_ORDER_NEAREST,
+                           cand.p, nullptr, nullptr, h->stream);
         if (rs == LQG_OK) {
             hipError_t e = build_run_iteration(b, p, h->stream);
             if (e != hipSuccess) rs = hip_fail(e, "build_run_iteration");
@@ -234,6 +246,9 @@ lqg_status build_index(lqg_index* h, const float* data, uint32_t ef_build, uint3
     }
     hipError_t e = hipStreamSynchronize(h->stream);
     if (rs == LQG_OK && e != hipSuccess) rs = hip_fail(e, "build synchronize");
+    if (fs_cand) h2_build_end(h);
+    rot.release();
+    out_k1.release();
     if (rs == LQG_OK) {
         e = hipMemcpy(h->nbr.data(), h->d_nbr.p, n * g * 4, hipMemcpyDeviceToHost);
         if (e != hipSuccess) rs = hip_fail(e, "graph download");

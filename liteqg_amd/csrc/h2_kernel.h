@@ -658,6 +658,8 @@ __device__ bool h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
             failed = true;
             break;
         }
+        if (job.exp_ids && n_exp < job.exp_cap && lane == 0)
+            job.exp_ids[static_cast<size_t>(q) * job.exp_cap + n_exp] = cur;
         ++n_exp;
 
         const unsigned char* rec = ix.rec + static_cast<size_t>(cur) * ix.stride;

@@ -398,17 +398,25 @@ hipError_t h2_launch_probe(const H2Dev& ix, const float* queries, uint32_t nq,
     return hipGetLastError();
 }
 
-hipError_t h2_launch_sidecar(const H2Dev& ix, float* rot, hipStream_t stream) {
+hipError_t h2_launch_rotate_rows(const H2Dev& ix, float* rot, hipStream_t stream) {
     const uint32_t waves = 4;
     hipLaunchKernelGGL(h2_rotate_kernel, dim3((ix.n + waves - 1) / waves), dim3(kWave * waves),
                        static_cast<size_t>(waves) * ix.p * 4, stream, ix, rot);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+hipError_t h2_launch_codes(const H2Dev& ix, const float* rot, hipStream_t stream) {
     const double fn = static_cast<double>(1.0f / sqrtf(static_cast<float>(ix.p)));
     const uint64_t threads = static_cast<uint64_t>(ix.n) * ix.deg;
     hipLaunchKernelGGL(h2_sidecar_kernel, dim3(static_cast<uint32_t>((threads + 255) / 256)),
                        dim3(256), 0, stream, ix, rot, fn);
     return hipGetLastError();
+}
+
+hipError_t h2_launch_sidecar(const H2Dev& ix, float* rot, hipStream_t stream) {
+    hipError_t e = h2_launch_rotate_rows(ix, rot, stream);
+    if (e != hipSuccess) return e;
+    return h2_launch_codes(ix, rot, stream);
 }
 
 hipError_t h2_launch_rotate_queries(const H2Dev& ix, const float* queries, uint32_t nq,

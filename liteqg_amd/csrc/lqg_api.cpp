@@ -811,6 +811,15 @@ lqg_status lqg_get_graph(const lqg_index* h, uint32_t* neighbors) {
     return LQG_OK;
 }
 
+lqg_status lqg_set_build_candidates(lqg_index* h, lqg_build_candidates source) {
+    if (!h) return fail(LQG_ERR_ARG, "null index");
+    const int v = lqg::abi_enum(source);
+    if (v != LQG_BUILD_SQ8 && v != LQG_BUILD_FASTSCAN)
+        return fail(LQG_ERR_ARG, "candidate source must be LQG_BUILD_SQ8 or LQG_BUILD_FASTSCAN");
+    h->build_candidates = v;
+    return LQG_OK;
+}
+
 lqg_status lqg_build_index(lqg_index* h, const float* data, uint64_t num_points,
                            uint64_t dimension, uint32_t ef_build, uint32_t num_iter,
                            uint32_t num_thread) {

@@ -98,6 +98,20 @@ uint64_t splitmix64(uint64_t& s) {
     return z ^ (z >> 31);
 }
 
+// FHTRotator draws (2 * bernoulli - 1) / sqrt(P) per sign (utils/rotator.hpp:37-43);
+// here the bits come from a seeded splitmix64 stream.
+std::vector<float> draw_signs(uint32_t p, uint64_t seed) {
+    std::vector<float> sg(p);
+    uint64_t s = seed;
+    uint64_t bits = 0;
+    for (uint32_t i = 0; i < p; ++i) {
+        if (i % 64 == 0) bits = splitmix64(s);
+        const int b = static_cast<int>((bits >> (i % 64)) & 1u);
+        sg[i] = static_cast<float>(2 * b - 1) / sqrtf(static_cast<float>(p));
+    }
+    return sg;
+}
+
 // Nibble of neighbour v for codebook c in the reference's FastScan block
 // layout (pack_codes / pack_codes_helper, fastscan_impl.hpp:67-124): blocks
 // of 32 neighbours, 4P bytes each; inside a block, 32 bytes per pair of
@@ -283,6 +297,94 @@ lqg_status run_h2_batch(lqg_index* h, const float* d_queries, uint64_t nq, const
     return LQG_OK;
 }
 
+// Graph build with FastScan candidates.  The reference's builder finds each
+// node's candidates with the FastScan search from the entry point, the node's
+// own vector as the query, and takes every node that search expands
+// (find_candidates, qg.hpp:439-473; qg_builder.hpp:202-231).  Here the
+// records of the graph being built carry codes and factors computed for the
+// current graph (SymphonyQG computes them as neighbour lists change; LiteQG's
+// rows carry none), and the general FastScan kernel writes each query's
+// expanded nodes in expansion order.
+//
+// h2_build_begin: records with the raw rows and the initial graph's ids,
+// rotation signs from `seed`, and every raw row rotated once into rot (n x P;
+// the same rows the search's query preparation would compute).
+lqg_status h2_build_begin(lqg_index* h, uint64_t seed, DevBuf<float>& rot) {
+    lqg_status st = set_layout(h);
+    if (st != LQG_OK) return st;
+    h->h2_signs = draw_signs(h->h2_p, seed);
+    st = alloc_records(h);
+    if (st != LQG_OK) return st;
+    LQG_HIP_TRY(rot.reserve(h->n * static_cast<size_t>(h->h2_p)));
+    LQG_HIP_TRY(h2_launch_rotate_rows(h->h2dev(), rot.p, h->stream));
+    return LQG_OK;
+}
+
+// Every query searched at ef with the general kernel, the first exp_cap
+// nodes each expands written to exp_ids (nq x exp_cap, 0xFFFFFFFF past the
+// last).  rot: the queries rotated beforehand (nq x P), or null.
+lqg_status h2_run_expansions(lqg_index* h, const float* d_queries, uint64_t nq, const float* rot,
+                             uint32_t ef, uint32_t* exp_ids, uint32_t exp_cap,
+                             DevBuf<uint32_t>& out_k1, hipStream_t s) {
+    const bool saved = h->h2_generic;
+    h->h2_generic = true;
+    H2Cfg cfg;
+    lqg_status st = h2_batch_cfg(h, nq, ef, 1, &cfg);
+    const H2Dev ix = h->h2dev();
+    h->h2_generic = saved;
+    if (st != LQG_OK) return st;
+    Scratch* sc = &h->scratch;
+    st = h2_reserve(h, cfg, nq, sc);
+    if (st != LQG_OK) return st;
+    LQG_HIP_TRY(out_k1.reserve(nq));
+    LQG_HIP_TRY(hipMemsetAsync(exp_ids, 0xFF, nq * exp_cap * 4ull, s));
+    const uint32_t gbits = sc->no_overflow ? 0u : h->h2_gvis_bits;
+    H2Job job;
+    job.queries = d_queries;
+    job.ef = nullptr;
+    job.k = nullptr;
+    job.nq = static_cast<uint32_t>(nq);
+    job.ef_all = ef;
+    job.k_all = 1;
+    job.kstride = 1;
+    job.out_ids = out_k1.p;
+    job.out_d = nullptr;
+    job.out_stats = nullptr;
+    job.counter = sc->counter.p;
+    job.gvis = gbits ? sc->gvis.p : nullptr;
+    job.gvis_bits = gbits;
+    job.ovf = gbits ? sc->ovf.p : nullptr;
+    job.err = sc->err.p;
+    job.rot = rot;
+    job.exp_ids = exp_ids;
+    job.exp_cap = exp_cap;
+    LQG_HIP_TRY(launch_reset_counter(sc->counter.p, s));
+    LQG_HIP_TRY(launch_reset_counter(sc->err.p, s));
+    if (job.ovf) LQG_HIP_TRY(launch_reset_counter(job.ovf, s, 2));
+    LQG_HIP_TRY(h2_launch_search(ix, job, cfg, s));
+    if (job.ovf) LQG_HIP_TRY(h2_launch_overflow(ix, job, cfg, overflow_waves(nq), s));
+    return LQG_OK;
+}
+
+// One iteration's candidates: the records take the current device graph's
+// rows, their codes and factors are recomputed, then every node is searched
+// at ef from the entry point, its own rotated row as the query, and its
+// first exp_cap expanded nodes written to exp_ids.  d_raw: the n x dim raw
+// rows on the device (the queries).
+lqg_status h2_build_candidates(lqg_index* h, const float* d_raw, const float* rot, uint32_t ef,
+                               uint32_t* exp_ids, uint32_t exp_cap, DevBuf<uint32_t>& out_k1,
+                               hipStream_t s) {
+    {
+        const H2Dev ix = h->h2dev();
+        LQG_HIP_TRY(h2_launch_rows(ix, h->h2_off_ids, h->d_nbr.p, 4u * ix.deg, true, s));
+        LQG_HIP_TRY(h2_launch_codes(ix, rot, s));
+    }
+    h->h2_ready = true;
+    return h2_run_expansions(h, d_raw, h->n, rot, ef, exp_ids, exp_cap, out_k1, s);
+}
+
+void h2_build_end(lqg_index* h) { h2_reset(h); }
+
 lqg_status h2_check_error(lqg_index* h, Scratch* sc) {
     if (!sc) sc = &h->scratch;
     if (h->mode != LQG_MODE_FASTSCAN || !sc->err.p) return LQG_OK;
@@ -335,14 +437,7 @@ lqg_status lqg_h2_build_sidecar(lqg_index* h, const float* signs, uint64_t seed)
             sg[i] = signs[i];
         }
     } else {
-        // FHTRotator draws (2 * bernoulli - 1) / sqrt(P) (utils/rotator.hpp:37-43)
-        uint64_t s = seed;
-        uint64_t bits = 0;
-        for (uint32_t i = 0; i < p; ++i) {
-            if (i % 64 == 0) bits = lqg::splitmix64(s);
-            const int b = static_cast<int>((bits >> (i % 64)) & 1u);
-            sg[i] = static_cast<float>(2 * b - 1) / sqrtf(static_cast<float>(p));
-        }
+        sg = lqg::draw_signs(p, seed);
     }
     h->h2_signs = sg;
     st = lqg::alloc_records(h);
@@ -511,6 +606,29 @@ lqg_status lqg_h2_load_sidecar(lqg_index* h, const char* path) {
     fclose(f);
     if (!ok) return fail(LQG_ERR_IO, std::string("short read from ") + path);
     h->h2_ready = true;
+    return LQG_OK;
+}
+
+lqg_status lqg_h2_expansions(lqg_index* h, const float* queries, uint64_t nq, uint32_t ef,
+                             uint32_t cap, uint32_t* out) {
+    if (!h || !queries || !out) return fail(LQG_ERR_ARG, "null argument");
+    if (nq == 0 || nq > 0x7FFFFFFFull) return fail(LQG_ERR_ARG, "nq must be in [1, 2^31)");
+    if (cap == 0 || cap > 65536) return fail(LQG_ERR_ARG, "cap must be in [1, 65536]");
+    if (ef == 0 || lqg::ef_class(ef) == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024");
+    if (!h->h2_ready) return fail(LQG_ERR_STATE, "no sidecar: build, set or load one first");
+    for (uint64_t i = 0; i < nq * h->dim; ++i)
+        if (!std::isfinite(queries[i])) return fail(LQG_ERR_ARG, "query contains a non-finite value");
+    lqg::DevBuf<float> dq;
+    lqg::DevBuf<uint32_t> dout, k1;
+    LQG_HIP_TRY(dq.reserve(nq * h->dim));
+    LQG_HIP_TRY(dout.reserve(nq * cap));
+    LQG_HIP_TRY(hipMemcpy(dq.p, queries, nq * h->dim * 4, hipMemcpyHostToDevice));
+    lqg_status st = lqg::h2_run_expansions(h, dq.p, nq, nullptr, ef, dout.p, cap, k1, h->stream);
+    if (st != LQG_OK) return st;
+    LQG_HIP_TRY(hipStreamSynchronize(h->stream));
+    st = lqg::h2_check_error(h);
+    if (st != LQG_OK) return st;
+    LQG_HIP_TRY(hipMemcpy(out, dout.p, nq * cap * 4ull, hipMemcpyDeviceToHost));
     return LQG_OK;
 }
 

@@ -145,6 +145,7 @@ struct lqg_index {
     bool h2_launch_set[lqg::kClasses] = {};
     lqg_rotation h2_rotation = LQG_ROTATION_FHT;
     bool h2_generic = false;     // lqg_h2_set_generic
+    int build_candidates = 0;    // lqg_build_candidates of build_index
     uint32_t h2_gvis_bits = 14;  // per-wave overflow visited table, log2 entries (0: none)
 
     lqg::H2Dev h2dev() const {
@@ -219,6 +220,18 @@ void h2_reset(lqg_index* h);
 // After a synchronised FastScan launch with scratch sc (null: the handle's):
 // fails if a visited set overflowed.
 lqg_status h2_check_error(lqg_index* h, Scratch* sc = nullptr);
+
+// Graph build with FastScan candidates (lqg_h2.cpp): records and rotated rows
+// once, then per iteration the expanded nodes of every node's search on the
+// current graph; h2_build_end drops the build's records.
+lqg_status h2_build_begin(lqg_index* h, uint64_t seed, DevBuf<float>& rot);
+lqg_status h2_run_expansions(lqg_index* h, const float* d_queries, uint64_t nq, const float* rot,
+                             uint32_t ef, uint32_t* exp_ids, uint32_t exp_cap,
+                             DevBuf<uint32_t>& out_k1, hipStream_t s);
+lqg_status h2_build_candidates(lqg_index* h, const float* d_raw, const float* rot, uint32_t ef,
+                               uint32_t* exp_ids, uint32_t exp_cap, DevBuf<uint32_t>& out_k1,
+                               hipStream_t s);
+void h2_build_end(lqg_index* h);
 
 // Device graph build for lqg_build_index.
 lqg_status build_index(lqg_index* h, const float* data, uint32_t ef_build, uint32_t num_iter);

@@ -119,6 +119,12 @@ struct H2Job {
     // indices), zeroed before launch; null: a query whose LDS visited table
     // would pass half full fails instead.
     uint32_t* ovf = nullptr;
+    // Graph build (find_candidates, qg.hpp:439-473): the ids of the first
+    // exp_cap nodes each query expands, in expansion order, to exp_ids[q *
+    // exp_cap ...] (slots past its expansions keep what the caller put
+    // there).  General kernel only; null in searches.
+    uint32_t* exp_ids = nullptr;
+    uint32_t exp_cap = 0;
 };
 
 struct H2Cfg {
@@ -163,6 +169,10 @@ hipError_t h2_launch_probe(const H2Dev& ix, const float* queries, uint32_t nq,
 // Sidecar build: rotates every raw vector, then fills codes and factors of
 // every record (raw and ids must already be in the records).
 hipError_t h2_launch_sidecar(const H2Dev& ix, float* rot_scratch, hipStream_t stream);
+// The two halves of h2_launch_sidecar: the records' raw rows rotated into rot
+// (n x p), then codes and factors of every record from rot and its ids.
+hipError_t h2_launch_rotate_rows(const H2Dev& ix, float* rot, hipStream_t stream);
+hipError_t h2_launch_codes(const H2Dev& ix, const float* rot, hipStream_t stream);
 // Copies w bytes per node between a contiguous device array (n x w) and
 // byte offset `off` of every record (to_rec: into the records).
 hipError_t h2_launch_rows(const H2Dev& ix, uint32_t off, void* buf, uint32_t w, bool to_rec,

@@ -29,7 +29,8 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import MODE, ORDER, ROTATION, Launch, LqgError, Stats, check, lib, ptr
+from ._lib import (BUILD_CANDIDATES, MODE, ORDER, ROTATION, Launch, LqgError, Stats, check, lib,
+                   ptr)
 
 UINT_MAX = 0xFFFFFFFF
 
@@ -78,12 +79,21 @@ class Index:
         check(lib().lqg_set_ef(self._h, int(EF)))
 
     def build_index(self, data, EF: int, num_iter: int = 3,  # noqa: N803
-                    num_thread: int = UINT_MAX) -> None:
+                    num_thread: int = UINT_MAX, *, candidates: str | None = None) -> None:
+        """Build the graph on the device (bindings.cpp:69-90).  candidates:
+        ``"sq8"`` (default: the nearest entries of an SQ8 beam search) or
+        ``"fastscan"`` (the reference's find_candidates: every node a FastScan
+        search over the graph being built expands); None keeps the current
+        setting."""
         arr = np.ascontiguousarray(data, dtype=np.float32)
         if arr.ndim == 1:
             arr = arr.reshape(1, -1)
         if arr.ndim != 2:
             raise ValueError("Input data has an incorrect shape. Data must be a 1D or 2D array.")
+        if candidates is not None:
+            if candidates not in BUILD_CANDIDATES:
+                raise ValueError(f"candidates must be 'sq8' or 'fastscan', got {candidates!r}")
+            check(lib().lqg_set_build_candidates(self._h, BUILD_CANDIDATES[candidates]))
         check(lib().lqg_build_index(self._h, ptr(arr), arr.shape[0], arr.shape[1], int(EF),
                                     int(num_iter), int(min(num_thread, UINT_MAX))))
 
@@ -174,6 +184,15 @@ class Index:
         check(lib().lqg_set_sq8_integer(self._h, 1 if enable else 0))
 
     # -- FastScan RaBitQ path (search_qg) ----------------------------------
+    def fastscan_expansions(self, queries, ef: int, cap: int) -> np.ndarray:
+        """Diagnostic: the first ``cap`` nodes each query's FastScan search
+        expands at ``ef``, in order (0xFFFFFFFF past the last): the candidate
+        list of the reference's find_candidates (qg.hpp:439-473)."""
+        Q = np.ascontiguousarray(queries, dtype=np.float32).reshape(-1, self.dimension)
+        out = np.empty((Q.shape[0], int(cap)), np.uint32)
+        check(lib().lqg_h2_expansions(self._h, ptr(Q), Q.shape[0], int(ef), int(cap), ptr(out)))
+        return out
+
     def set_overflow_bits(self, bits: int) -> None:
         """log2 entries of the per-wave device visited table a FastScan query
         moves to once its LDS table is half full (default 14); 0 = none, the

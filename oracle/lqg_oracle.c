@@ -29,6 +29,12 @@
  *                          two code layouts give identical sums.
  *   orc_search_h2       -- ids against the reference's search_qg on rebuilt
  *                          rows (reference sidecar), every ef of the fixture.
+ *   orc_h2_expansions_batch -- the same traversal's expansion order; pinned
+ *                          through orc_search_h2 (its pops and expansions are
+ *                          the ones that decide the pinned ids) and by the
+ *                          invariants of its tests (the list's length is E,
+ *                          distinct ids, the first is the entry point, every
+ *                          later one is a neighbour of an earlier one).
  *   orc_sidecar_build   -- codes bit-exact against rabitq_codes; factors to
  *                          2e-6 of their magnitude (FP64 sums vs float).
  *
@@ -671,8 +677,40 @@ typedef struct {
 /* update_results (qg.hpp:395-418) and copy_results (buffer.hpp:102):        */
 /* results are the k nearest expanded nodes by exact distance; slots never  */
 /* filled hold 0, as the zero-initialised ResultBuffer copies them.         */
+static int h2_search_impl(const orc_h2* ix, const float* query, uint32_t ef, uint32_t k,
+                          uint32_t* results, float* dists, uint64_t* stats, uint32_t* exp,
+                          uint32_t exp_cap);
+
 int orc_search_h2(const orc_h2* ix, const float* query, uint32_t ef, uint32_t k,
                   uint32_t* results, float* dists, uint64_t* stats) {
+    return h2_search_impl(ix, query, ef, k, results, dists, stats, NULL, 0);
+}
+
+/* find_candidates (qg.hpp:439-473): the same search from the entry point,    */
+/* and the nodes it expands, in expansion order (the builder keeps them as a */
+/* node's candidates, dropping the node itself).  out: nq x cap ids, the     */
+/* first cap expansions of each query, 0xFFFFFFFF past its last.             */
+int orc_h2_expansions_batch(const orc_h2* ix, const float* queries, uint64_t nq, uint32_t ef,
+                            uint32_t cap, uint32_t* out, int threads) {
+    int err = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 4) reduction(| : err)
+#endif
+    for (int64_t q = 0; q < (int64_t)nq; ++q) {
+        uint32_t one;
+        uint32_t* o = out + (size_t)q * cap;
+        for (uint32_t i = 0; i < cap; ++i) o[i] = 0xFFFFFFFFu;
+        err |= h2_search_impl(ix, queries + (size_t)q * ix->dim, ef, 1, &one, NULL, NULL, o,
+                              cap) != 0;
+    }
+    (void)threads;
+    return err ? -1 : 0;
+}
+
+static int h2_search_impl(const orc_h2* ix, const float* query, uint32_t ef, uint32_t k,
+                          uint32_t* results, float* dists, uint64_t* stats, uint32_t* exp,
+                          uint32_t exp_cap) {
     if (ef == 0 || k == 0) return -1;
     uint32_t p = ix->p, deg = ix->deg, dim = ix->dim;
     float* rq = (float*)malloc(4 * (size_t)p);
@@ -696,6 +734,7 @@ int orc_search_h2(const orc_h2* ix, const float* query, uint32_t ef, uint32_t k,
         pops++;
         if (bs_get(&vis, cur)) continue;
         bs_set(&vis, cur);
+        if (exp && E < exp_cap) exp[E] = cur;
         E++;
         const float* row = ix->raw + (size_t)cur * dim;
         float sqr_y = orc_l2_sqr(query, row, dim);

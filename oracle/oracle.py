@@ -270,6 +270,7 @@ def _h2sigs():
     L.orc_buffers_replay.argtypes = [P, C.c_uint64, C.c_uint32, P, P, P]
     L.orc_search_h2.argtypes = [P, P, C.c_uint32, C.c_uint32, P, P, P]
     L.orc_search_h2_batch.argtypes = [P, P, C.c_uint64, C.c_uint32, C.c_uint32, P, P, P, C.c_int]
+    L.orc_h2_expansions_batch.argtypes = [P, P, C.c_uint64, C.c_uint32, C.c_uint32, P, C.c_int]
     L.orc_sidecar_build.argtypes = [P, P, P, C.c_int]
     L.orc_codes_to_vector_major.argtypes = [P, C.c_uint32, C.c_uint32, P]
     L._h2 = True
@@ -355,6 +356,17 @@ def codes_to_vector_major(ref_codes, deg, p):
     L = _h2sigs()
     for i in range(ref_codes.shape[0]):
         L.orc_codes_to_vector_major(_p(ref_codes[i]), deg, p, _p(out[i]))
+    return out
+
+
+def h2_expansions_batch(h: H2Arrays, queries, ef: int, cap: int, threads: int = 0):
+    """find_candidates (qg.hpp:439-473): the first cap nodes the FastScan search
+    of each query expands, in order (0xFFFFFFFF past the last)."""
+    Q = np.ascontiguousarray(queries, np.float32)
+    out = np.empty((Q.shape[0], cap), np.uint32)
+    if _h2sigs().orc_h2_expansions_batch(h.ref, _p(Q), Q.shape[0], ef, cap, _p(out),
+                                         threads) != 0:
+        raise RuntimeError("orc_h2_expansions_batch failed")
     return out
 
 

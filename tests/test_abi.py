@@ -66,8 +66,10 @@ def test_keyword_names_match_reference():
     assert list(inspect.signature(Index.load).parameters)[1:] == ["filename"]
     assert list(inspect.signature(Index.save).parameters)[1:] == ["filename"]
     assert list(inspect.signature(Index.set_ef).parameters)[1:] == ["EF"]
-    assert list(inspect.signature(Index.build_index).parameters)[1:] == [
-        "data", "EF", "num_iter", "num_thread"]
+    # the reference's four parameters, in order; additions are keyword-only
+    bp = inspect.signature(Index.build_index).parameters
+    assert list(bp)[1:5] == ["data", "EF", "num_iter", "num_thread"]
+    assert all(v.kind is inspect.Parameter.KEYWORD_ONLY for v in list(bp.values())[5:])
     assert list(inspect.signature(Index.search).parameters)[1:] == ["query", "k"]
     import symphonyqg
 

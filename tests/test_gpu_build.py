@@ -12,12 +12,12 @@ from conftest import GOLDEN
 pytestmark = pytest.mark.gpu
 
 
-def built_index(base, deg=32, ef=100, iters=3):
+def built_index(base, deg=32, ef=100, iters=3, candidates="sq8"):
     from liteqg_amd.symphonyqg import Index
 
     n, d = base.shape
     ix = Index("QG", "L2", n, d, deg)
-    ix.build_index(base, ef, num_iter=iters)
+    ix.build_index(base, ef, num_iter=iters, candidates=candidates)
     return ix
 
 
@@ -86,16 +86,18 @@ def test_build_is_repeatable_with_a_hub():
     assert indeg[0] > 1024, "the origin should be a hub"
 
 
+@pytest.mark.parametrize("candidates", ["sq8", "fastscan"])
 @pytest.mark.parametrize("name,n,d,deg", [("g_uniform_n2000_d64", 2000, 64, 32),
                                           ("g_intmix_n1200_d128", 1200, 128, 32),
                                           ("g_unitmix_n1000_d96_deg64", 1000, 96, 64)])
-def test_recall_not_below_reference_build(oracle_mod, name, n, d, deg):
-    """Same data as a reference-built fixture: our graph's recall is not lower."""
+def test_recall_not_below_reference_build(oracle_mod, name, n, d, deg, candidates):
+    """Same data as a reference-built fixture: our graph's recall is not lower,
+    with either candidate source."""
     ref = oracle_mod.read_index(f"{GOLDEN}/{name}.index", n, d, deg)
     base = ref.raw.copy()
     z = np.load(f"{GOLDEN}/{name}.npz")
     Q = z["queries"]
-    ix = built_index(base, deg=deg, ef=200)
+    ix = built_index(base, deg=deg, ef=200, candidates=candidates)
     check_rows(ix.graph(), n)
     gt, _ = oracle_mod.bruteforce_fp64(base, Q, 10)
     for ef in (10, 20, 50):
@@ -103,6 +105,41 @@ def test_recall_not_below_reference_build(oracle_mod, name, n, d, deg):
         theirs_ids, _, _ = oracle_mod.search_canon_batch(ref, Q, ef, 10, "nearest")
         theirs = oracle_mod.recall_at_k(theirs_ids, gt, 10)
         assert mine >= theirs - 0.02, (ef, mine, theirs)
+
+
+def test_fastscan_candidates_build(oracle_mod, tmp_path):
+    """Graph build with the reference's candidate source (every node a FastScan
+    search over the graph being built expands): valid rows, repeatable,
+    recall on a clustered 128-d set; afterwards the index has no sidecar left
+    over from the build and the SQ8 search on it equals the oracle's."""
+    from liteqg_amd import datasets
+    from liteqg_amd.symphonyqg import LqgError
+
+    base, Q = datasets.make("int_mix", 20000, 128, 200, seed=5, centres=64, sigma=20.0)
+    ix = built_index(base, ef=200, candidates="fastscan")
+    g = ix.graph()
+    check_rows(g, 20000)
+    assert np.array_equal(built_index(base, ef=200, candidates="fastscan").graph(), g)
+    assert recall(oracle_mod, ix, base, Q, 100) > 0.95
+    with pytest.raises(LqgError):
+        ix.set_mode("fastscan")  # the build's records are gone
+    ix.save(str(tmp_path / "f.index"))
+    arr = oracle_mod.read_index(str(tmp_path / "f.index"), 20000, 128, 32)
+    e_ids, _, _ = oracle_mod.search_canon_batch(arr, Q[:50], 100, 10, "nearest")
+    assert np.array_equal(ix.search_batch(Q[:50], 10, ef=100, order="nearest"), e_ids)
+
+
+@pytest.mark.parametrize("kind,d,deg,kw", [
+    ("unit_mix", 768, 32, dict(centres=64, sigma=0.25, centre_law="normal")),
+    ("pos_mix", 960, 64, dict(centres=32, sigma=0.05)),
+])
+def test_fastscan_candidates_build_high_dimension(oracle_mod, kind, d, deg, kw):
+    from liteqg_amd import datasets
+
+    base, Q = datasets.make(kind, 6000, d, 100, seed=4, **kw)
+    ix = built_index(base, deg=deg, ef=200, candidates="fastscan")
+    check_rows(ix.graph(), 6000)
+    assert recall(oracle_mod, ix, base, Q, 100) > 0.9
 
 
 def test_build_clustered_int_data(oracle_mod):

@@ -123,6 +123,41 @@ def test_update_results_matches_reference(oracle_mod, name):
 
 
 @pytest.mark.parametrize("name", H2_NAMES)
+def test_expansions_match_oracle(oracle_mod, name):
+    """The expanded-node lists the FastScan-candidate graph build takes
+    (find_candidates, qg.hpp:439-473) equal the oracle's, in order, on the
+    reference's sidecar; a cap below the count keeps the first entries."""
+    arr, _ = load_golden(name)
+    h = np.load(f"{GOLDEN}/h2_{name}.npz")
+    ix = _index_from_golden(name)
+    ix.set_sidecar(h["signs"], h["codes"], h["factors"], layout="blocks")
+    H = oracle_mod.H2Arrays(arr, h["signs"], h["codes"], h["factors"], 0)
+    for ef in (10, 50, 100, 400):
+        got = ix.fastscan_expansions(h["queries"], ef, 1024)
+        assert np.array_equal(got, oracle_mod.h2_expansions_batch(H, h["queries"], ef, 1024)), ef
+    assert np.array_equal(ix.fastscan_expansions(h["queries"], 50, 9),
+                          oracle_mod.h2_expansions_batch(H, h["queries"], 50, 9))
+
+
+@pytest.mark.parametrize("d,deg", [(128, 32), (960, 64)])
+def test_expansions_match_oracle_device_sidecar(oracle_mod, tmp_path, d, deg):
+    arr, base = make_random_index(1200 if d < 900 else 500, d, deg, seed=d, knn=True)
+    ix = _index_from_arrays(arr, tmp_path, "e")
+    ix.build_sidecar(seed=2)
+    H = _h2arr(oracle_mod, arr, ix)
+    rng = np.random.default_rng(4)
+    Q = (base[rng.integers(0, arr.n, 40)] + rng.normal(0, 0.02, (40, d))).astype(np.float32)
+    for ef in (20, 200):
+        assert np.array_equal(ix.fastscan_expansions(Q, ef, 960),
+                              oracle_mod.h2_expansions_batch(H, Q, ef, 960)), ef
+
+
+def _h2arr(O, arr, ix):
+    s, codes, fac = ix.get_sidecar()
+    return O.H2Arrays(arr, s, codes, fac, 1)
+
+
+@pytest.mark.parametrize("name", H2_NAMES)
 def test_device_sidecar_matches_oracle_and_reference(oracle_mod, name):
     """The device-built sidecar equals the oracle's; searching with it gives
     the reference's ids on the fixtures."""

@@ -216,6 +216,36 @@ def test_update_results_matches_reference(oracle_mod, name):
 
 
 @pytest.mark.parametrize("name", H2_NAMES)
+def test_expansion_list_invariants(oracle_mod, name):
+    """find_candidates (qg.hpp:439-473) keeps the nodes search_qg's traversal
+    expands, in order.  The list's length is the expansion count of the
+    pinned search (same traversal), its ids are distinct, it starts at the
+    entry point, and every later id is a neighbour of an earlier one (a node
+    enters the buffer only as a neighbour of an expanded node)."""
+    ix, h = _h2(name)
+    arr = oracle_mod.H2Arrays(ix, h["signs"], h["codes"], h["factors"], 0)
+    q = h["queries"][:40]
+    for ef in (10, 50, 100):
+        ex = oracle_mod.h2_expansions_batch(arr, q, ef, 1024)
+        _, _, st = oracle_mod.search_h2_batch(arr, q, ef, 10)
+        for r in range(q.shape[0]):
+            e = ex[r][ex[r] != 0xFFFFFFFF]
+            assert e.size == st[r, 0]
+            assert np.all(ex[r][e.size:] == 0xFFFFFFFF)
+            assert len(set(e.tolist())) == e.size
+            assert e[0] == ix.ep
+            seen = {int(e[0])}
+            nb = set(ix.nbr[e[0]].tolist())
+            for x in e[1:]:
+                assert int(x) in nb
+                seen.add(int(x))
+                nb |= set(ix.nbr[x].tolist())
+    # a cap below the expansion count keeps the first cap of them
+    full = oracle_mod.h2_expansions_batch(arr, q, 50, 1024)
+    assert np.array_equal(oracle_mod.h2_expansions_batch(arr, q, 50, 7), full[:, :7])
+
+
+@pytest.mark.parametrize("name", H2_NAMES)
 def test_sidecar_codes_equal_reference(oracle_mod, name):
     """rabitq_codes (quantization/rabitq.hpp:23-59): this build's sidecar has
     the reference's sign codes bit for bit (in vector-major layout)."""
