@@ -610,6 +610,127 @@ h1_search_kernel_w6(DevIndex ix, SearchJob job, uint32_t cache_bits, uint32_t di
     search_kernel_body<EF_CAP, EVAL>(ix, job, cache_bits, dim_pad, wave_bytes);
 }
 
+// ---------------------------------------------------------------------------
+// Latency mode, for launches of a few queries (a lone Index.search call): one
+// query per workgroup of two waves.  Wave 0 walks the query with search_one,
+// the code every batch kernel runs, so results are the same.  Wave 1 only
+// loads: it watches wave 0's pool mirror in LDS (the stage array, rewritten
+// at every merge) and fetches ahead what the next expansions will read, so
+// that wave 0 finds it in the L2 instead of waiting on HBM twice per
+// expansion (the neighbour-id row, then the candidates' SQ8 rows):
+//   depth 1: the id rows of the two best unexpanded pool entries;
+//   depth 2: also every 128-byte line of the SQ8 rows of their neighbours;
+//   depth 3: the same for the four best unexpanded entries.
+// The node wave 0 is expanding is one of the two (the mirror shows it
+// unexpanded until the next merge); the other is the runner-up, which is the
+// next node expanded unless the current expansion admits a closer one.
+// Nothing wave 1 does reaches wave 0 except through the caches.  It stops
+// when wave 0 raises its done flag, or after kPrefetchRounds rounds.
+constexpr uint32_t kPrefetchRounds = 1u << 16;
+
+__device__ void prefetch_ahead(const DevIndex& ix, const uint64_t* stage, uint32_t ef_cap,
+                               const uint32_t* done, uint32_t depth, uint32_t lane,
+                               uint32_t* sink_out) {
+    uint32_t sink = 0;
+    // the last eight nodes fetched (scalar registers)
+    uint32_t hist[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) hist[i] = kInvalid;
+    const uint32_t deg = min(ix.deg, 64u);
+    // nodes fetched per round: 2, or 4 at depth 3; a node's id row is read by
+    // 64 / T lanes, each taking ids c, c + 64 / T, ...
+    const uint32_t tcount = depth >= 3 ? 4u : 2u;
+    const uint32_t per = kWave / tcount;
+    for (uint32_t r = 0; r < kPrefetchRounds; ++r) {
+        if (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u) break;
+        uint64_t key = ~0ull;
+        if (lane < ef_cap)
+            key = __hip_atomic_load(stage + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t id = key_id(key);
+        bool open = (key & 1ull) == 0ull && id < ix.n;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) open = open && id != hist[i];
+        uint64_t m = __ballot(open);
+        if (m == 0) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        // the first tcount open entries, best first; this lane's node
+        uint32_t node = kInvalid;
+        for (uint32_t t = 0; t < tcount && m != 0; ++t) {
+            const uint32_t x = uniform(__shfl(id, static_cast<int>(__builtin_ctzll(m))));
+            m &= m - 1;
+#pragma unroll
+            for (int i = 7; i > 0; --i) hist[i] = hist[i - 1];
+            hist[0] = x;
+            if (lane / per == t) node = x;
+        }
+        uint32_t nid[4] = {kInvalid, kInvalid, kInvalid, kInvalid};
+        if (node < ix.n) {
+            const uint32_t* row = ix.nbr + static_cast<size_t>(node) * ix.deg;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                const uint32_t c = lane % per + j * per;
+                if (c < deg) nid[j] = row[c];
+            }
+        }
+        sink += nid[0] ^ nid[1] ^ nid[2] ^ nid[3];
+        if (depth >= 2) {
+            uint32_t v = 0;
+            for (uint32_t o = 0; o < ix.dim; o += 128u)
+#pragma unroll
+                for (uint32_t j = 0; j < 4; ++j)
+                    if (nid[j] < ix.n) v += ix.qdata[static_cast<size_t>(nid[j]) * ix.dim + o];
+            sink += v;
+        }
+    }
+    // keeps the loads: the condition depends on every value they returned
+    if (sink == 0x9E3779B9u) *sink_out = sink;
+}
+
+template <int EF_CAP, int EVAL>
+__global__ void __launch_bounds__(128) h1_latency_kernel(DevIndex ix, SearchJob job,
+                                                         uint32_t cache_bits, uint32_t dim_pad,
+                                                         uint32_t wave_bytes, uint32_t depth) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wid = threadIdx.x / kWave;
+    constexpr bool kParams = EVAL == EVAL_FLOAT;
+    float* pmin = nullptr;
+    float* pscale = nullptr;
+    if constexpr (kParams) {
+        pmin = reinterpret_cast<float*>(lds);
+        pscale = pmin + dim_pad;
+        for (uint32_t i = threadIdx.x; i < ix.dim; i += blockDim.x) {
+            pmin[i] = ix.sq8_min[i];
+            pscale[i] = ix.sq8_scale[i];
+        }
+    }
+    unsigned char* wb = lds + (kParams ? 8u * dim_pad : 0u);
+    WaveLds w;
+    w.qv = reinterpret_cast<float*>(wb);
+    w.stage = reinterpret_cast<uint64_t*>(wb + 4u * dim_pad);
+    w.bids = reinterpret_cast<uint32_t*>(w.stage + EF_CAP);
+    w.cache = w.bids + kWave;
+    uint32_t* done = reinterpret_cast<uint32_t*>(wb + wave_bytes);
+    // the mirror starts empty (keys ~0 read as expanded)
+    for (uint32_t i = threadIdx.x; i < static_cast<uint32_t>(EF_CAP); i += blockDim.x)
+        w.stage[i] = ~0ull;
+    if (threadIdx.x == 0) {
+        done[0] = 0u;
+        done[1] = 0u;
+    }
+    __syncthreads();
+    const uint32_t q = blockIdx.x;
+    if (wid == 0) {
+        if (q < job.nq)
+            search_one<EF_CAP, EVAL>(ix, job, q, pmin, pscale, w, cache_bits, lane, nullptr);
+        if (lane == 0) __hip_atomic_store(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+        prefetch_ahead(ix, w.stage, static_cast<uint32_t>(EF_CAP), done, depth, lane, done + 1);
+    }
+}
+
 __global__ void reset_counter_kernel(uint32_t* counter, uint32_t n) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) counter[i] = 0u;
 }
@@ -640,6 +761,27 @@ KernelFn select_for_eval(int ef_cap, int occ) {
         case 1024: return kernel_for<1024, EVAL>(occ);
         default: return nullptr;
     }
+}
+
+using LatencyFn = void (*)(DevIndex, SearchJob, uint32_t, uint32_t, uint32_t, uint32_t);
+
+template <int EVAL>
+LatencyFn latency_for_eval(int ef_cap) {
+    switch (ef_cap) {
+        case 64: return &h1_latency_kernel<64, EVAL>;
+        case 128: return &h1_latency_kernel<128, EVAL>;
+        case 192: return &h1_latency_kernel<192, EVAL>;
+        case 256: return &h1_latency_kernel<256, EVAL>;
+        case 512: return &h1_latency_kernel<512, EVAL>;
+        case 1024: return &h1_latency_kernel<1024, EVAL>;
+        default: return nullptr;
+    }
+}
+
+LatencyFn select_latency(const LaunchCfg& cfg) {
+    if (cfg.eval == EVAL_INT128) return latency_for_eval<EVAL_INT128>(cfg.ef_cap);
+    return cfg.eval == EVAL_INT ? latency_for_eval<EVAL_INT>(cfg.ef_cap)
+                                : latency_for_eval<EVAL_FLOAT>(cfg.ef_cap);
 }
 
 KernelFn select_kernel(const LaunchCfg& cfg) {
@@ -689,6 +831,26 @@ int search_groups(const DevIndex& ix, const SearchJob& job, const LaunchCfg& cfg
 
 hipError_t launch_search(const DevIndex& ix, const SearchJob& job, const LaunchCfg& cfg,
                          hipStream_t stream) {
+    if (cfg.prefetch > 0 && job.nq <= kLatencyMaxQueries && !job.visit_bits) {
+        // latency mode: one query per workgroup (search wave + prefetch wave)
+        LatencyFn k = select_latency(cfg);
+        if (!k || cfg.cache_bits < 4 || cfg.cache_bits > 14 || cfg.prefetch > 3)
+            return hipErrorInvalidValue;
+        uint32_t dim_pad, wave_bytes;
+        geometry(ix, cfg, dim_pad, wave_bytes);
+        const size_t lds = (cfg.eval == EVAL_FLOAT ? 8u * static_cast<size_t>(dim_pad) : 0u) +
+                           wave_bytes + 16u;
+        if (lds > 64 * 1024) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               static_cast<int>(lds));
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k, dim3(job.nq), dim3(2 * kWave), lds, stream, ix, job,
+                           static_cast<uint32_t>(cfg.cache_bits), dim_pad, wave_bytes,
+                           static_cast<uint32_t>(cfg.prefetch));
+        return hipGetLastError();
+    }
     KernelFn k = select_kernel(cfg);
     if (!k || cfg.cache_bits < 4 || cfg.cache_bits > 14 || cfg.waves_per_group < 1 ||
         cfg.waves_per_group > 4)

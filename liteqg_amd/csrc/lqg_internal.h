@@ -83,7 +83,14 @@ struct LaunchCfg {
     int occupancy;       // register cap: 8 = 64 VGPRs, 6 = 80 VGPRs, 0 = uncapped
     int eval = 0;        // 0: SQ8 float distances; 1: exact integer distances (v_dot4);
                          // 3: the same for rows of at most 128 bytes
+    int prefetch = 0;    // launches of at most kLatencyMaxQueries queries: 0 = the batch
+                         // kernel; 1..3 = latency mode with that prefetch depth
 };
+
+// Largest launch that takes the latency-mode kernel (one query per
+// workgroup, a prefetch wave beside each search wave) when its class's
+// prefetch depth is set.
+constexpr uint32_t kLatencyMaxQueries = 256;
 
 // Device view of an index for the FastScan RaBitQ search (search_qg,
 // qg.hpp:252-287).  One record per node, `stride` bytes (a multiple of 128):

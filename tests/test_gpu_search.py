@@ -122,6 +122,34 @@ def test_random_index_parity(oracle_mod, tmp_path, case):
         assert st["evaluations"] >= int(est[:, 1].sum())
 
 
+@pytest.mark.parametrize("prefetch", [1, 2, 3])
+@pytest.mark.parametrize("kind,d,deg,ef", [("int", 128, 32, 50), ("uniform", 96, 64, 300),
+                                           ("uniform", 768, 32, 100)])
+def test_latency_mode_matches_oracle(oracle_mod, tmp_path, prefetch, kind, d, deg, ef):
+    """Latency mode (one query per workgroup, a second wave loading ahead):
+    batches of up to 256 queries and the plain per-query search give the
+    oracle's ids and distance bits in both orders; a larger batch runs the
+    batch kernel with the same results."""
+    arr, base = make_random_index(3000, d, deg, seed=d + prefetch, kind=kind, knn=True)
+    ix = device_from_arrays(arr, tmp_path)
+    ix.set_launch(ef, waves_per_group=1, cache_bits=10, prefetch=prefetch)
+    assert ix.get_launch(ef)["prefetch"] == prefetch
+    rng = np.random.default_rng(5)
+    Q = (base[rng.integers(0, 3000, 300)] + rng.normal(0, 0.05 if kind == "uniform" else 10,
+                                                       size=(300, d))).astype(np.float32)
+    if kind == "int":
+        Q = np.clip(np.rint(Q), 0, 255).astype(np.float32)
+    for order in ("reference", "nearest"):
+        for nq in (1, 37, 256, 300):
+            ids, dis = ix.search_batch(Q[:nq], 10, ef=ef, order=order, return_distances=True)
+            exp, exp_d, _ = oracle_batch(oracle_mod, arr, Q[:nq], ef, 10, order)
+            assert_same(ids, dis, exp, exp_d)
+    ix.set_ef(ef)
+    ix.set_order("nearest")
+    exp, _, _ = oracle_batch(oracle_mod, arr, Q[:20], ef, 10, "nearest")
+    assert np.array_equal(np.stack([ix.search(q, 10) for q in Q[:20]]), exp)
+
+
 def test_uneven_ef_and_k_in_one_launch(oracle_mod, tmp_path):
     arr, base = make_random_index(8000, 64, 32, seed=5, knn=True)
     ix = device_from_arrays(arr, tmp_path)
