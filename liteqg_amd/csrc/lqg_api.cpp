@@ -735,6 +735,11 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
     const int n_occ = cls <= 256 ? 4 : 3;
     const int bits[] = {8, 9, 10, 11};
     const int wpg[] = {1, 2, 4};
+    // A tuning set larger than a latency-mode launch times the batch kernel
+    // only, and keeps the class's prefetch depth for small launches (a lone
+    // search() after tuning on a large batch still runs in latency mode).
+    const bool small = nq <= lqg::kLatencyMaxQueries;
+    const int keep_pf = had ? saved.prefetch : lqg::default_launch(cls).prefetch;
     std::vector<lqg::LaunchCfg> cands;
     for (int oi = 0; oi < n_occ; ++oi)
         for (int cb : bits)
@@ -743,13 +748,13 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
                 c.occupancy = occs[oi];
                 c.cache_bits = cb;
                 c.waves_per_group = wg;
-                c.prefetch = 0;
+                c.prefetch = small ? 0 : keep_pf;
                 cands.push_back(c);
             }
     // a tuning set as small as a latency-mode launch also tries that mode at
     // each prefetch depth (one query per workgroup: waves per group and the
     // register cap do not apply)
-    if (nq <= lqg::kLatencyMaxQueries)
+    if (small)
         for (int pf : {1, 2, 3})
             for (int cb : bits) {
                 lqg::LaunchCfg c = lqg::default_launch(cls);

@@ -150,6 +150,26 @@ def test_latency_mode_matches_oracle(oracle_mod, tmp_path, prefetch, kind, d, de
     assert np.array_equal(np.stack([ix.search(q, 10) for q in Q[:20]]), exp)
 
 
+def test_autotune_and_prefetch_depth(oracle_mod, tmp_path):
+    """The default prefetch depth of small launches survives tuning on a large
+    batch (which times the batch kernel only); tuning on a small set picks a
+    depth among the latency-mode ones or the batch kernel; results stay the
+    oracle's throughout."""
+    arr, base = make_random_index(3000, 64, 32, seed=8, knn=True)
+    ix = device_from_arrays(arr, tmp_path)
+    rng = np.random.default_rng(9)
+    Q = (base[rng.integers(0, 3000, 400)] + rng.normal(0, 0.05, (400, 64))).astype(np.float32)
+    assert ix.get_launch(50)["prefetch"] == 2
+    big = ix.autotune(Q, 50, 10)
+    assert big["prefetch"] == 2 and ix.get_launch(50)["prefetch"] == 2
+    small = ix.autotune(Q[:64], 50, 10)
+    assert small["prefetch"] in (0, 1, 2, 3)
+    exp, exp_d, _ = oracle_batch(oracle_mod, arr, Q, 50, 10, "nearest")
+    for nq in (1, 64, 400):
+        ids, dis = ix.search_batch(Q[:nq], 10, ef=50, order="nearest", return_distances=True)
+        assert_same(ids, dis, exp[:nq], exp_d[:nq])
+
+
 def test_uneven_ef_and_k_in_one_launch(oracle_mod, tmp_path):
     arr, base = make_random_index(8000, 64, 32, seed=5, knn=True)
     ix = device_from_arrays(arr, tmp_path)
