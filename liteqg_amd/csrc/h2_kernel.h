@@ -262,9 +262,19 @@ __device__ __forceinline__ float exact_l2_regs(const float (&xr)[kRegJ], const f
         const float t = qr[u] - xr[u];
         s = fmaf(t, t, s);
     }
-    for (uint32_t j = kRegJ; j < nj; ++j) {
-        const float t = w.qv[p + 16 * j] - row[p + 16 * j];
-        s = fmaf(t, t, s);
+    // dims past the registers' (D > 16 kRegJ), 16 per partial at a time: a
+    // block's loads are issued together, then its FMAs run in dim order
+    for (uint32_t j0 = kRegJ; j0 < nj; j0 += 16u) {
+        float xv[16];
+#pragma unroll
+        for (uint32_t u = 0; u < 16u; ++u) xv[u] = j0 + u < nj ? row[p + 16u * (j0 + u)] : 0.0f;
+#pragma unroll
+        for (uint32_t u = 0; u < 16u; ++u) {
+            if (j0 + u < nj) {
+                const float t = w.qv[p + 16u * (j0 + u)] - xv[u];
+                s = fmaf(t, t, s);
+            }
+        }
     }
     s = s + dpp_shl(s, 8);
     s = s + dpp_shl(s, 4);
