@@ -622,6 +622,18 @@ def fastscan_leg(ix, db, Q, gt, k, args, ctx):
             ix.build_sidecar(seed=args.seed)  # back to the device-built sidecar
     par["ok"] = all(r["ids_equal"] and r["dist_bits_equal"] for r in par["vs_oracle"]) and \
         par.get("vs_reference_search_qg", {}).get("rows_equal_up_to_ties", 1.0) == 1.0
+    # the plain path in this mode: one Index.search call per query
+    ix.set_ef(chosen)
+    lat = []
+    for q in Q[:args.plain]:
+        t0 = time.perf_counter()
+        ix.search(q, k)
+        lat.append((time.perf_counter() - t0) * 1e3)
+    lat = np.array(lat)
+    plain = {"qps": round(len(lat) / (lat.sum() / 1e3), 1),
+             "p50_ms": round(float(np.median(lat)), 4),
+             "p99_ms": round(float(np.percentile(lat, 99)), 4), "queries": len(lat)}
+    log(f"fastscan plain path: {plain}")
     ix.set_mode("sq8")
     log(f"fastscan timed ef={chosen}: {ms_step:.3f} ms/step, {qps:.0f} QPS, "
         f"{achieved:.0f} GB/s")
@@ -629,6 +641,7 @@ def fastscan_leg(ix, db, Q, gt, k, args, ctx):
             "recall_at_10": rec, "qps": round(qps, 1), "ms_per_step": round(ms_step, 4),
             "ef_sweep": sweep, "sidecar_build_s": round(side_s, 2), "launch": tuned,
             "gemm_rotation": gemm, "steady_state": steady, "parity": par,
+            "plain_gpu": plain, "batched_over_plain": round(qps / plain["qps"], 1),
             "cpu_baseline": cpu,
             "gpu_over_cpu": round(qps / cpu["value"], 1) if cpu else None,
             "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
