@@ -267,15 +267,15 @@ typedef struct {
                                 64 VGPRs), for indexes with P / lanes per
                                 neighbour <= 128 code bits (8: k <= 64)          */
     int32_t groups;          /* workgroups of the persistent grid; 0 = fill the device  */
-    int32_t prefetch;        /* SQ8 launches of at most 256 queries (a lone search()):
+    int32_t prefetch;        /* launches of at most 256 queries (a lone search()):
                                 0 = the batch kernel; 1, 2 or 3 = latency mode, one
                                 query per workgroup with a second wave that loads
-                                ahead what the next expansions read -- 1: the id
-                                rows of the two best unexpanded pool entries, 2:
-                                also their neighbours' SQ8 rows, 3: both for the
-                                four best unexpanded entries.  Results
-                                are the same; only time changes.  FastScan:
-                                ignored (0)                                         */
+                                ahead what the next expansions read.  SQ8 -- 1: the
+                                id rows of the two best unexpanded pool entries,
+                                2: also their neighbours' SQ8 rows, 3: both for
+                                the four best.  FastScan: the node records of the
+                                2^prefetch best.  Results are the same; only time
+                                changes.  Default 2                                 */
 } lqg_launch;
 
 /* Sets the launch settings used for every ef in the class of `ef` (ef = 0:

@@ -500,6 +500,30 @@ __device__ __forceinline__ void h2_fast_body(const H2Dev& ix, const H2Job& job,
     }
 }
 
+// Latency mode of the specialised kernel (as h2_latency_kernel).
+template <int EF_CAP, int NJ, int KS>
+__global__ void __launch_bounds__(128) h2_fast_latency_kernel(H2Dev ix, H2Job job,
+                                                               uint32_t hash_bits, Layout lay) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wid = threadIdx.x / kWave;
+    const H2Lds w = carve(lds, lay);
+    uint32_t* flag = reinterpret_cast<uint32_t*>(lds + lay.wave_bytes);
+    if (threadIdx.x == 0) {
+        flag[0] = 0u;
+        flag[1] = 0u;
+    }
+    __syncthreads();
+    if (wid == 0) {
+        if (blockIdx.x < job.nq)
+            (void)h2_fast_one<EF_CAP, NJ, KS>(ix, job, blockIdx.x, w, hash_bits, lane);
+        if (lane == 0) __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+        h2_prefetch_ahead(ix, w.st, static_cast<uint32_t>(EF_CAP), flag, job.prefetch_targets,
+                          lane, flag + 1);
+    }
+}
+
 template <int EF_CAP, int NJ, int KS>
 __global__ void __launch_bounds__(256) h2_fast_kernel(H2Dev ix, H2Job job, uint32_t hash_bits,
                                                        Layout lay) {

@@ -12,6 +12,11 @@ using SearchFn = void (*)(H2Dev, H2Job, uint32_t, Layout);
 
 template <int EF_CAP, int NJ>
 SearchFn fast_for(int ks, int occ) {
+    if (occ == kOccLatency) {
+        if (ks == 1) return &h2_fast_latency_kernel<EF_CAP, NJ, 1>;
+        if (ks == 2) return &h2_fast_latency_kernel<EF_CAP, NJ, 2>;
+        return nullptr;
+    }
     if (ks == 1) {
         switch (occ) {
             case 0: return &h2_fast_kernel<EF_CAP, NJ, 1>;

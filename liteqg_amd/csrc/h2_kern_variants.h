@@ -14,6 +14,17 @@ using SearchFn = void (*)(H2Dev, H2Job, uint32_t, Layout);
 
 template <int EF_CAP, int KS>
 SearchFn variant_for_cw(int cw, int occ) {
+    if (occ == kOccLatency) {
+        switch (cw) {
+            case 1: return &h2_latency_kernel<EF_CAP, 1, KS>;
+            case 2: return &h2_latency_kernel<EF_CAP, 2, KS>;
+            case 4: return &h2_latency_kernel<EF_CAP, 4, KS>;
+            case 8: return &h2_latency_kernel<EF_CAP, 8, KS>;
+            case 16: return &h2_latency_kernel<EF_CAP, 16, KS>;
+            case 32: return &h2_latency_kernel<EF_CAP, 32, KS>;
+            default: return nullptr;
+        }
+    }
     if (occ == 6) {
         switch (cw) {
             case 1: return &h2_search_kernel_w6<EF_CAP, 1, KS>;

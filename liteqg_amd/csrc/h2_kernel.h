@@ -856,6 +856,97 @@ __device__ __forceinline__ void h2_search_body(const H2Dev& ix, const H2Job& job
     }
 }
 
+// ---------------------------------------------------------------------------
+// Latency mode (launches of a few queries: a lone Index.search): one query
+// per workgroup of two waves.  Wave 0 runs the search of the batch kernels
+// (same results).  Wave 1 only loads: it reads wave 0's pool staging records
+// in LDS (rewritten at every merge; an entry is unexpanded when bit 0 of its
+// key and the top bit of its id are clear, which covers both kernels' flag
+// conventions) and fetches every 128-byte line of the records of the first
+// `targets` unexpanded entries, so that the next expansion finds its record
+// in the L2 instead of waiting on HBM.  Nothing it does reaches wave 0 except
+// through the caches; it stops on wave 0's done flag in LDS, or after
+// kH2PrefetchRounds rounds.
+constexpr uint32_t kH2PrefetchRounds = 1u << 16;
+
+__device__ void h2_prefetch_ahead(const H2Dev& ix, const uint4* st, uint32_t ef_cap,
+                                  const uint32_t* done, uint32_t targets, uint32_t lane,
+                                  uint32_t* sink_out) {
+    uint32_t sink = 0;
+    uint32_t hist[8];  // the last eight nodes fetched (scalar registers)
+#pragma unroll
+    for (int i = 0; i < 8; ++i) hist[i] = kEmpty;
+    targets = min(max(targets, 1u), 8u);
+    const uint32_t lines = (ix.stride + 127u) / 128u;
+    const uint32_t* stw = reinterpret_cast<const uint32_t*>(st);
+    for (uint32_t r = 0; r < kH2PrefetchRounds; ++r) {
+        if (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u) break;
+        uint32_t klo = 1u, id = kEmpty;
+        if (lane < ef_cap) {
+            klo = __hip_atomic_load(stw + 4u * lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            id = __hip_atomic_load(stw + 4u * lane + 2u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        bool open = (klo & 1u) == 0u && id < ix.n;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) open = open && id != hist[i];
+        uint64_t m = __ballot(open);
+        if (m == 0) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        uint32_t tg[8];
+        uint32_t cnt = 0;
+#pragma unroll
+        for (uint32_t t = 0; t < 8u; ++t) {
+            tg[t] = kEmpty;
+            if (t < targets && m != 0) {
+                const uint32_t x = uniform(__shfl(id, static_cast<int>(__builtin_ctzll(m))));
+                m &= m - 1;
+#pragma unroll
+                for (int i = 7; i > 0; --i) hist[i] = hist[i - 1];
+                hist[0] = x;
+                tg[t] = x;
+                cnt = t + 1u;
+            }
+        }
+        for (uint32_t i = lane; i < cnt * lines; i += kWave) {
+            const uint32_t t = i / lines, line = i - t * lines;
+            uint32_t node = tg[0];
+#pragma unroll
+            for (uint32_t u = 1; u < 8u; ++u) node = t == u ? tg[u] : node;
+            sink += *reinterpret_cast<const uint32_t*>(ix.rec + static_cast<size_t>(node) * ix.stride +
+                                                       128u * line);
+        }
+    }
+    // keeps the loads: the condition depends on every value they returned
+    if (sink == 0x9E3779B9u) *sink_out = sink;
+}
+
+template <int EF_CAP, int CW, int KS>
+__global__ void __launch_bounds__(128) h2_latency_kernel(H2Dev ix, H2Job job, uint32_t hash_bits,
+                                                          Layout lay) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wid = threadIdx.x / kWave;
+    const H2Lds w = carve(lds, lay);
+    uint32_t* flag = reinterpret_cast<uint32_t*>(lds + lay.wave_bytes);
+    if (threadIdx.x == 0) {
+        flag[0] = 0u;
+        flag[1] = 0u;
+    }
+    __syncthreads();
+    if (wid == 0) {
+        if (blockIdx.x < job.nq)
+            (void)h2_search_one<EF_CAP, CW, KS, false>(ix, job, blockIdx.x, w, hash_bits, nullptr,
+                                                       lane);
+        if (lane == 0) __hip_atomic_store(flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+        h2_prefetch_ahead(ix, w.st, static_cast<uint32_t>(EF_CAP), flag, job.prefetch_targets,
+                          lane, flag + 1);
+    }
+}
+
 // The overflow kernel: searches again, with the visited set in a per-wave
 // device table, the queries the search kernel put on the overflow list
 // (run after it on the same stream; it finds an empty list at once).

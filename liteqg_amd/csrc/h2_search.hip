@@ -350,6 +350,25 @@ int h2_groups(const H2Dev& ix, const H2Cfg& cfg, uint32_t kmax, uint32_t nq) {
 
 hipError_t h2_launch_search(const H2Dev& ix, const H2Job& job, const H2Cfg& cfg,
                             hipStream_t stream) {
+    if (cfg.prefetch > 0 && job.nq <= kLatencyMaxQueries && job.prefetch_targets > 0 &&
+        !job.exp_ids) {
+        // latency mode: one query per workgroup (search wave + prefetch wave)
+        SearchFn k = select_search(ix, cfg.ef_cap, kOccLatency, job.kstride);
+        if (!k || cfg.hash_bits < 6 || cfg.hash_bits > 14 || job.kstride == 0)
+            return hipErrorInvalidValue;
+        const Layout lay = search_layout(ix, cfg);
+        const size_t lds = lay.wave_bytes + 16u;
+        if (lds > 160 * 1024) return hipErrorInvalidValue;
+        if (lds > 64 * 1024) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               static_cast<int>(lds));
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k, dim3(job.nq), dim3(2 * kWave), lds, stream, ix, job,
+                           static_cast<uint32_t>(cfg.hash_bits), lay);
+        return hipGetLastError();
+    }
     SearchFn k = select_search(ix, cfg.ef_cap, cfg.occupancy, job.kstride);
     if (!k || cfg.waves_per_group < 1 || cfg.waves_per_group > 4 || cfg.hash_bits < 6 ||
         cfg.hash_bits > 14 || job.kstride == 0)

@@ -607,6 +607,7 @@ lqg_status lqg_set_launch(lqg_index* h, uint32_t ef, const lqg_launch* cfg) {
             c.hash_bits = cfg->cache_bits;
             c.occupancy = cfg->occupancy;
             c.groups = cfg->groups;
+            c.prefetch = cfg->prefetch;
             h->h2_launch[slot] = c;
             h->h2_launch_set[slot] = true;
             continue;
@@ -639,7 +640,7 @@ lqg_status lqg_get_launch(const lqg_index* h, uint32_t ef, lqg_launch* cfg) {
         cfg->cache_bits = c.hash_bits;
         cfg->occupancy = c.occupancy;
         cfg->groups = c.groups;
-        cfg->prefetch = 0;
+        cfg->prefetch = c.prefetch;
         return LQG_OK;
     }
     const lqg::LaunchCfg c = h->launch_set[slot] ? h->launch[slot] : lqg::default_launch(cls);
@@ -674,6 +675,10 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
         const lqg::H2Cfg base = lqg::h2_default_launch(cls);
         float best_ms = 1e30f;
         lqg::H2Cfg best_cfg = base;
+        // a tuning set as small as a latency-mode launch also tries that mode
+        // at each prefetch depth; a larger one keeps the class's depth
+        const bool small = nq <= lqg::kLatencyMaxQueries;
+        const int keep_pf = had ? saved.prefetch : base.prefetch;
         std::vector<lqg::H2Cfg> cands;
         for (int occ : {0, 6, 8}) {
             if (!h->h2_ready || !lqg::h2_variant_ok(h->h2dev(), occ, k)) continue;
@@ -683,9 +688,18 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
                     c.occupancy = occ;
                     c.waves_per_group = wg;
                     c.hash_bits = base.hash_bits + extra;
+                    c.prefetch = small ? 0 : keep_pf;
                     if (c.hash_bits >= 6) cands.push_back(c);
                 }
         }
+        if (small && h->h2_ready)
+            for (int pf : {1, 2, 3})
+                for (int extra = -1; extra < 2; ++extra) {
+                    lqg::H2Cfg c = base;
+                    c.hash_bits = base.hash_bits + extra;
+                    c.prefetch = pf;
+                    if (c.hash_bits >= 6) cands.push_back(c);
+                }
         for (const lqg::H2Cfg& c : cands) {
             h->h2_launch[slot] = c;
             h->h2_launch_set[slot] = true;
@@ -722,7 +736,7 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
             best->cache_bits = best_cfg.hash_bits;
             best->occupancy = best_cfg.occupancy;
             best->groups = best_cfg.groups;
-            best->prefetch = 0;
+            best->prefetch = best_cfg.prefetch;
         }
         return LQG_OK;
     }

@@ -132,6 +132,9 @@ struct H2Job {
     // there).  General kernel only; null in searches.
     uint32_t* exp_ids = nullptr;
     uint32_t exp_cap = 0;
+    // Latency mode (launches of at most kLatencyMaxQueries queries): records
+    // of the best unexpanded pool entries a query's prefetch wave loads ahead.
+    uint32_t prefetch_targets = 0;
 };
 
 struct H2Cfg {
@@ -140,7 +143,14 @@ struct H2Cfg {
     int groups;          // persistent grid size (0 = fill the device)
     int hash_bits;       // log2 entries of the per-wave LDS visited table
     int occupancy = 0;   // 0 = uncapped registers, 6 or 8 = capped for that many waves per SIMD
+    int prefetch = 0;    // launches of at most kLatencyMaxQueries queries: 0 = the batch
+                         // kernel; d = 1..3: latency mode, a prefetch wave per query loading
+                         // the records of the 2^d best unexpanded pool entries
 };
+
+// Kernel-table key of the FastScan latency-mode kernels (in place of a
+// register cap).
+constexpr int kOccLatency = -2;
 
 // Returns LDS bytes one workgroup needs for this configuration.
 size_t search_lds_bytes(const DevIndex& ix, const LaunchCfg& cfg);

@@ -350,6 +350,33 @@ def test_update_results_and_uneven_ef_k(oracle_mod, tmp_path):
         assert np.all(ids[i, k[i]:kmax] == 0xFFFFFFFF)
 
 
+@pytest.mark.parametrize("prefetch", [1, 2, 3])
+@pytest.mark.parametrize("n,d,deg,ef", [(1500, 128, 32, 128), (1000, 96, 64, 300),
+                                        (400, 960, 64, 200)])
+def test_latency_mode_matches_oracle(oracle_mod, tmp_path, prefetch, n, d, deg, ef):
+    """FastScan latency mode (one query per workgroup, a second wave loading
+    the records of the best unexpanded entries ahead; the specialised kernel
+    at D 128 / degree 32, the general one otherwise): batches of up to 256
+    queries and the plain per-query search give the oracle's ids and distance
+    bits; a larger batch runs the batch kernel with the same results."""
+    arr, base = make_random_index(n, d, deg, seed=d + prefetch, knn=True)
+    ix = _index_from_arrays(arr, tmp_path, "lat")
+    ix.build_sidecar(seed=3)
+    ix.set_mode("fastscan")
+    ix.set_launch(ef, waves_per_group=1, cache_bits=10, prefetch=prefetch)
+    assert ix.get_launch(ef)["prefetch"] == prefetch
+    rng = np.random.default_rng(6)
+    Q = (base[rng.integers(0, n, 300)] + rng.normal(0, 0.02, (300, d))).astype(np.float32)
+    H = _h2arr(oracle_mod, arr, ix)
+    exp, exp_d, _ = oracle_mod.search_h2_batch(H, Q, ef, 10)
+    for nq in (1, 37, 256, 300):
+        ids, dis = ix.search_batch(Q[:nq], 10, ef=ef, return_distances=True)
+        assert np.array_equal(ids, exp[:nq]), nq
+        assert np.array_equal(_bits(dis), _bits(exp_d[:nq])), nq
+    ix.set_ef(ef)
+    assert np.array_equal(np.stack([ix.search(q, 10) for q in Q[:10]]), exp[:10])
+
+
 def test_visited_overflow_table(oracle_mod, tmp_path):
     """A 64-entry LDS visited table moves to device memory after 32
     expansions; results do not change."""
