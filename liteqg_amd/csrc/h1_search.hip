@@ -63,6 +63,72 @@ __device__ __forceinline__ void acc1(uint32_t byte, float qq, float mn, float sc
     s = fmaf(t, t, s);
 }
 
+// NB chunks of 16 dims from chunk `base` of an SQ8 row into a lane's PL
+// partials (lane t of its group holds partials PL t .. PL t + PL - 1, i.e.
+// bytes PL t .. of every chunk).  All NB loads are issued before any
+// arithmetic, and the block carries no branch: a branch per chunk kept each
+// chunk's LDS reads of the query and the SQ8 parameters from being issued
+// ahead of the previous chunk's arithmetic, exposing their latency once per
+// 16 dims.  MASK: chunks past the row (base + c >= nchunk) read the last
+// chunk and add t = 0, i.e. fma(0, 0, s) = s exactly.
+template <int PL, int NB, bool MASK>
+__device__ __forceinline__ void chunk_block(const uint8_t* row, uint32_t t, const float* qv,
+                                            const float* pmin, const float* pscale, uint32_t base,
+                                            uint32_t nchunk, float (&s)[PL]) {
+    constexpr int W = PL >= 4 ? PL / 4 : 1;  // 32-bit words per chunk and lane
+    uint32_t wv[NB][W];
+    uint32_t ch[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        ch[c] = MASK ? min(base + c, nchunk - 1u) : base + c;
+        const uint8_t* src = row + 16 * ch[c] + PL * t;
+        if constexpr (PL == 1) {
+            wv[c][0] = *src;
+        } else if constexpr (PL == 2) {
+            wv[c][0] = *reinterpret_cast<const uint16_t*>(src);
+        } else if constexpr (PL == 8) {
+            const uint2 v = *reinterpret_cast<const uint2*>(src);
+            wv[c][0] = v.x;
+            wv[c][1] = v.y;
+        } else {
+            wv[c][0] = *reinterpret_cast<const uint32_t*>(src);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        const bool ok = !MASK || base + c < nchunk;
+        const uint32_t b = 16 * ch[c] + PL * t;
+        if constexpr (PL < 4) {
+#pragma unroll
+            for (int p = 0; p < PL; ++p) {
+                const float x = fmaf(static_cast<float>((wv[c][0] >> (8 * p)) & 0xFFu),
+                                     pscale[b + p], pmin[b + p]);
+                float tt = qv[b + p] - x;
+                if (MASK) tt = ok ? tt : 0.0f;
+                s[p] = fmaf(tt, tt, s[p]);
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < W; ++v) {
+                const float4 qq = *reinterpret_cast<const float4*>(qv + b + 4 * v);
+                const float4 mn = *reinterpret_cast<const float4*>(pmin + b + 4 * v);
+                const float4 sc = *reinterpret_cast<const float4*>(pscale + b + 4 * v);
+                const uint32_t x = wv[c][v];
+                const float q4[4] = {qq.x, qq.y, qq.z, qq.w};
+                const float m4[4] = {mn.x, mn.y, mn.z, mn.w};
+                const float s4[4] = {sc.x, sc.y, sc.z, sc.w};
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float xd = fmaf(static_cast<float>((x >> (8 * e)) & 0xFFu), s4[e], m4[e]);
+                    float tt = q4[e] - xd;
+                    if (MASK) tt = ok ? tt : 0.0f;
+                    s[4 * v + e] = fmaf(tt, tt, s[4 * v + e]);
+                }
+            }
+        }
+    }
+}
+
 // Squared L2 of the query to candidate `id` by a group of L lanes (t = lane
 // index in the group).  Returns the full sum on every lane of the group.
 template <int L>
@@ -75,57 +141,51 @@ __device__ __forceinline__ float group_l2(const DevIndex& ix, const float* qv, c
     for (int p = 0; p < PL; ++p) s[p] = 0.f;
     const uint32_t nchunk = ix.dim >> 4;
     const uint8_t* row = ix.qdata + static_cast<size_t>(id) * ix.dim;
-    if (act) {
-        if ((ix.dim & 15u) == 0) {
-            // 16-byte aligned rows: PL bytes of each chunk in one load.  All loads
-            // of up to 8 chunks (128 dims) are issued before any arithmetic, so a
-            // row of <= 128 bytes costs one memory round trip.
-            constexpr int CB = 8;
-            constexpr int W = PL / 4;  // dwords per chunk per lane
-            for (uint32_t base = 0; base < nchunk; base += CB) {
-                uint32_t wv[CB][W];
+    // Chunks are taken in blocks of 8 (a row of <= 128 bytes is one memory
+    // round trip), then one block of the 6, 4 or 2 left, all without branches
+    // (see chunk_block); other remainders take a block of 8 with the chunks
+    // past the row masked.
+    if (act && (ix.dim & 15u) == 0) {
+        uint32_t base = 0;
+        for (; base + 8u <= nchunk; base += 8u)
+            chunk_block<PL, 8, false>(row, t, qv, pmin, pscale, base, nchunk, s);
+        const uint32_t rem = nchunk - base;
+        if (rem == 6u)
+            chunk_block<PL, 6, false>(row, t, qv, pmin, pscale, base, nchunk, s);
+        else if (rem == 4u)
+            chunk_block<PL, 4, false>(row, t, qv, pmin, pscale, base, nchunk, s);
+        else if (rem == 2u)
+            chunk_block<PL, 2, false>(row, t, qv, pmin, pscale, base, nchunk, s);
+        else if (rem != 0u)
+            chunk_block<PL, 8, true>(row, t, qv, pmin, pscale, base, nchunk, s);
+    } else if (act) {
+        for (uint32_t i = 0; i < nchunk; ++i)
 #pragma unroll
-                for (int c = 0; c < CB; ++c) {
-                    if (base + c < nchunk) {
-                        const uint8_t* src = row + 16 * (base + c) + PL * t;
-                        if constexpr (W == 2) {
-                            const uint2 v = *reinterpret_cast<const uint2*>(src);
-                            wv[c][0] = v.x;
-                            wv[c][1] = v.y;
-                        } else {
-                            wv[c][0] = *reinterpret_cast<const uint32_t*>(src);
-                        }
-                    }
-                }
-#pragma unroll
-                for (int c = 0; c < CB; ++c) {
-                    if (base + c < nchunk) {
-                        const uint32_t b = 16 * (base + c) + PL * t;
-#pragma unroll
-                        for (int v = 0; v < W; ++v) {
-                            const float4 qq = *reinterpret_cast<const float4*>(qv + b + 4 * v);
-                            const float4 mn = *reinterpret_cast<const float4*>(pmin + b + 4 * v);
-                            const float4 sc = *reinterpret_cast<const float4*>(pscale + b + 4 * v);
-                            const uint32_t x = wv[c][v];
-                            acc1(x & 0xFFu, qq.x, mn.x, sc.x, s[4 * v + 0]);
-                            acc1((x >> 8) & 0xFFu, qq.y, mn.y, sc.y, s[4 * v + 1]);
-                            acc1((x >> 16) & 0xFFu, qq.z, mn.z, sc.z, s[4 * v + 2]);
-                            acc1(x >> 24, qq.w, mn.w, sc.w, s[4 * v + 3]);
-                        }
-                    }
-                }
+            for (int p = 0; p < PL; ++p) {
+                const uint32_t d = 16 * i + PL * t + p;
+                acc1(row[d], qv[d], pmin[d], pscale[d], s[p]);
             }
-        } else {
-            for (uint32_t i = 0; i < nchunk; ++i)
-#pragma unroll
-                for (int p = 0; p < PL; ++p) {
-                    const uint32_t d = 16 * i + PL * t + p;
-                    acc1(row[d], qv[d], pmin[d], pscale[d], s[p]);
-                }
-        }
     }
     float r;
-    if constexpr (L == 4) {
+    if constexpr (L == 16) {
+        // lane t holds partial t: (p, p+8) across t^8, (p, p+4) across t^4, then
+        // (b0 + b2) across t^2 and (b0 + b2) + (b1 + b3) across t^1
+        s[0] += __shfl_xor(s[0], 8);
+        s[0] += __shfl_xor(s[0], 4);
+        s[0] += __shfl_xor(s[0], 2);
+        s[0] += __shfl_xor(s[0], 1);
+        r = s[0];
+    } else if constexpr (L == 8) {
+        // lane t holds partials 2t, 2t+1: (p, p+8) across t^4, (p, p+4) across
+        // t^2; lane 0 then holds b0, b1 and lane 1 b2, b3
+#pragma unroll
+        for (int p = 0; p < 2; ++p) s[p] += __shfl_xor(s[p], 4);
+#pragma unroll
+        for (int p = 0; p < 2; ++p) s[p] += __shfl_xor(s[p], 2);
+        const float c0 = s[0] + __shfl_xor(s[0], 1);
+        const float c1 = s[1] + __shfl_xor(s[1], 1);
+        r = c0 + c1;
+    } else if constexpr (L == 4) {
         // lane t holds partials 4t..4t+3: (p, p+8) across t^2, (p, p+4) across t^1
 #pragma unroll
         for (int p = 0; p < 4; ++p) s[p] += __shfl_xor(s[p], 2);
@@ -148,15 +208,28 @@ __device__ __forceinline__ float group_l2(const DevIndex& ix, const float* qv, c
 }
 
 // Keys of the m fresh candidates bids[0..m) (m <= 32) in one pass: candidate
-// c's key is returned on lane c*L (L = 4 for m <= 16, else 2); other lanes
-// return ~0.  All 64 lanes call it.
+// c's key is returned on lane c*L, with L = 64 / (the smallest of 4, 8, 16,
+// 32 that holds m) lanes per candidate, so that few candidates still spread
+// over the whole wave (each lane then carries fewer of the 16 partials, and
+// the dependent chains are shorter); other lanes return ~0.  All 64 lanes
+// call it.
 __device__ __forceinline__ uint64_t eval_fresh(const DevIndex& ix, const float* qv,
                                                const float* pmin, const float* pscale,
                                                const uint32_t* bids, uint32_t m,
                                                uint32_t lane) {
     float r;
     uint32_t c, L;
-    if (m <= 16) {
+    if (m <= 4) {
+        L = 16;
+        c = lane >> 4;
+        const bool act = c < m;
+        r = group_l2<16>(ix, qv, pmin, pscale, act ? bids[c] : 0u, act, lane & 15u);
+    } else if (m <= 8) {
+        L = 8;
+        c = lane >> 3;
+        const bool act = c < m;
+        r = group_l2<8>(ix, qv, pmin, pscale, act ? bids[c] : 0u, act, lane & 7u);
+    } else if (m <= 16) {
         L = 4;
         c = lane >> 2;
         const bool act = c < m;
