@@ -276,6 +276,13 @@ typedef struct {
                                 the four best.  FastScan: the node records of the
                                 2^prefetch best.  Results are the same; only time
                                 changes.  Default 2                                 */
+    int32_t split;           /* SQ8 float path (an index whose SQ8 decode is not the
+                                identity, or D > 256): waves per query sharing each
+                                expansion's distance evaluation, one query per
+                                workgroup; 0 or 1 = off, 2 or 4.  Takes precedence
+                                over prefetch.  Default: 2 for D >= 512 in launches
+                                of at most 4,096 queries, else off.
+                                FastScan: unused                                  */
 } lqg_launch;
 
 /* Sets the launch settings used for every ef in the class of `ef` (ef = 0:

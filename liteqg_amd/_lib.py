@@ -30,7 +30,8 @@ class LqgError(RuntimeError):
 
 class Launch(C.Structure):
     _fields_ = [("waves_per_group", C.c_int32), ("cache_bits", C.c_int32),
-                ("occupancy", C.c_int32), ("groups", C.c_int32), ("prefetch", C.c_int32)]
+                ("occupancy", C.c_int32), ("groups", C.c_int32), ("prefetch", C.c_int32),
+                ("split", C.c_int32)]
 
 
 class Stats(C.Structure):

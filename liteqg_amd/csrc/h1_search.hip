@@ -131,7 +131,7 @@ __device__ __forceinline__ void chunk_block(const uint8_t* row, uint32_t t, cons
 
 // Squared L2 of the query to candidate `id` by a group of L lanes (t = lane
 // index in the group).  Returns the full sum on every lane of the group.
-template <int L>
+template <int L, int NB>
 __device__ __forceinline__ float group_l2(const DevIndex& ix, const float* qv, const float* pmin,
                                          const float* pscale, uint32_t id, bool act,
                                          uint32_t t) {
@@ -141,16 +141,25 @@ __device__ __forceinline__ float group_l2(const DevIndex& ix, const float* qv, c
     for (int p = 0; p < PL; ++p) s[p] = 0.f;
     const uint32_t nchunk = ix.dim >> 4;
     const uint8_t* row = ix.qdata + static_cast<size_t>(id) * ix.dim;
-    // Chunks are taken in blocks of 8 (a row of <= 128 bytes is one memory
-    // round trip), then one block of the 6, 4 or 2 left, all without branches
-    // (see chunk_block); other remainders take a block of 8 with the chunks
-    // past the row masked.
+    // Chunks are taken in blocks of NB (8: a row of <= 128 bytes is one
+    // memory round trip), then a block of the 6, 4 or 2 left, all without
+    // branches (see chunk_block); other remainders take a block of 8, or of NB
+    // when more than 8 are left, with the chunks past the row masked.
     if (act && (ix.dim & 15u) == 0) {
         uint32_t base = 0;
-        for (; base + 8u <= nchunk; base += 8u)
-            chunk_block<PL, 8, false>(row, t, qv, pmin, pscale, base, nchunk, s);
+        for (; base + NB <= nchunk; base += NB)
+            chunk_block<PL, NB, false>(row, t, qv, pmin, pscale, base, nchunk, s);
         const uint32_t rem = nchunk - base;
-        if (rem == 6u)
+        if constexpr (NB > 8) {
+            if (rem > 8u) {
+                chunk_block<PL, NB, true>(row, t, qv, pmin, pscale, base, nchunk, s);
+                base = nchunk;
+            }
+        }
+        if (base == nchunk) {
+        } else if (rem == 8u)
+            chunk_block<PL, 8, false>(row, t, qv, pmin, pscale, base, nchunk, s);
+        else if (rem == 6u)
             chunk_block<PL, 6, false>(row, t, qv, pmin, pscale, base, nchunk, s);
         else if (rem == 4u)
             chunk_block<PL, 4, false>(row, t, qv, pmin, pscale, base, nchunk, s);
@@ -207,41 +216,42 @@ __device__ __forceinline__ float group_l2(const DevIndex& ix, const float* qv, c
     return r;
 }
 
+// log2 of the lanes per candidate when a wave evaluates m fresh candidates:
+// 64 / (the smallest of 4, 8, 16, 32 that holds m).
+__device__ __forceinline__ uint32_t fresh_lanes_log2(uint32_t m) {
+    return m <= 4 ? 4u : m <= 8 ? 3u : m <= 16 ? 2u : 1u;
+}
+
 // Keys of the m fresh candidates bids[0..m) (m <= 32) in one pass: candidate
-// c's key is returned on lane c*L, with L = 64 / (the smallest of 4, 8, 16,
-// 32 that holds m) lanes per candidate, so that few candidates still spread
-// over the whole wave (each lane then carries fewer of the 16 partials, and
-// the dependent chains are shorter); other lanes return ~0.  All 64 lanes
-// call it.
+// c's key is returned on lane c*L, L = 1 << fresh_lanes_log2(m), so that few
+// candidates still spread over the whole wave (each lane then carries fewer
+// of the 16 partials, and the dependent chains are shorter); other lanes
+// return ~0.  All 64 lanes call it.  NB: chunks per block of row loads (8 in
+// the batch kernels; more in the split kernel, whose waves hold few
+// candidates each and have registers to spare).
+template <int NB = 8>
 __device__ __forceinline__ uint64_t eval_fresh(const DevIndex& ix, const float* qv,
                                                const float* pmin, const float* pscale,
                                                const uint32_t* bids, uint32_t m,
                                                uint32_t lane) {
+    // blocks of NB chunks hold NB * max(1, PL / 4) row words per lane
+    constexpr int NB2 = NB > 16 ? NB / 2 : NB;
     float r;
-    uint32_t c, L;
-    if (m <= 4) {
-        L = 16;
-        c = lane >> 4;
-        const bool act = c < m;
-        r = group_l2<16>(ix, qv, pmin, pscale, act ? bids[c] : 0u, act, lane & 15u);
-    } else if (m <= 8) {
-        L = 8;
-        c = lane >> 3;
-        const bool act = c < m;
-        r = group_l2<8>(ix, qv, pmin, pscale, act ? bids[c] : 0u, act, lane & 7u);
-    } else if (m <= 16) {
-        L = 4;
-        c = lane >> 2;
-        const bool act = c < m;
-        r = group_l2<4>(ix, qv, pmin, pscale, act ? bids[c] : 0u, act, lane & 3u);
-    } else {
-        L = 2;
-        c = lane >> 1;
-        const bool act = c < m;
-        r = group_l2<2>(ix, qv, pmin, pscale, act ? bids[c] : 0u, act, lane & 1u);
-    }
-    const bool leader = c < m && (lane & (L - 1)) == 0;
-    return leader ? make_key(r, bids[c]) : ~0ull;
+    const uint32_t lg = fresh_lanes_log2(m);
+    const uint32_t L = 1u << lg;
+    const uint32_t c = lane >> lg;
+    const bool act = c < m;
+    const uint32_t id = act ? bids[c] : 0u;
+    if (L == 16)
+        r = group_l2<16, NB>(ix, qv, pmin, pscale, id, act, lane & 15u);
+    else if (L == 8)
+        r = group_l2<8, NB>(ix, qv, pmin, pscale, id, act, lane & 7u);
+    else if (L == 4)
+        r = group_l2<4, NB>(ix, qv, pmin, pscale, id, act, lane & 3u);
+    else
+        r = group_l2<2, NB2>(ix, qv, pmin, pscale, id, act, lane & 1u);
+    const bool leader = act && (lane & (L - 1)) == 0;
+    return leader ? make_key(r, id) : ~0ull;
 }
 
 // Exact integer evaluation.  When every SQ8 parameter is min 0 / scale 1 the
@@ -303,6 +313,42 @@ struct WaveLds {
     uint32_t* cache;  // visited cache (1 << cache_bits)
 };
 
+// Split evaluation (h1_split_kernel): a query's workgroup of W waves shares
+// each pass's distance evaluation.  Wave 0 walks the query; before each pass it
+// posts the fresh candidates (WaveLds::bids) and their count here, every wave
+// evaluates a share of them and writes their keys, and wave 0 takes the keys
+// back.  go = 0 tells the other waves that the query is done.
+struct SplitLds {
+    uint32_t go;
+    uint32_t m;
+    uint32_t pad[2];
+    uint64_t keys[kWave / 2];  // one per fresh candidate (at most 32 per pass)
+};
+
+// Chunks per block of row loads in the split kernel: 32 (one 32-bit word per
+// chunk and lane) for up to 4 partials per lane, 16 for 8 partials; half that
+// for pools of 512 or more entries, whose registers the pool takes.
+template <int EF_CAP>
+constexpr int split_nb() {
+    return EF_CAP >= 512 ? 16 : 32;
+}
+
+// Wave wid's share of a pass of m fresh candidates: candidates [c0, c0 + mw)
+// with c0 = wid * ceil(m / W), each key written to keys[c].
+template <int W, int NB>
+__device__ __forceinline__ void eval_share(const DevIndex& ix, const float* qv, const float* pmin,
+                                           const float* pscale, const uint32_t* bids,
+                                           uint32_t m, uint32_t wid, uint32_t lane,
+                                           uint64_t* keys) {
+    const uint32_t cpw = (m + W - 1) / W;
+    const uint32_t c0 = wid * cpw;
+    if (c0 >= m) return;
+    const uint32_t mw = min(cpw, m - c0);
+    const uint64_t key = eval_fresh<NB>(ix, qv, pmin, pscale, bids + c0, mw, lane);
+    const uint32_t lg = fresh_lanes_log2(mw);
+    if ((lane & ((1u << lg) - 1u)) == 0 && (lane >> lg) < mw) keys[c0 + (lane >> lg)] = key;
+}
+
 // Marks node x in the counting bitmap; true when it was not marked before.
 __device__ __forceinline__ bool first_visit(uint32_t* bits, uint32_t x) {
     return (atomicOr(&bits[x >> 5], 1u << (x & 31u)) & (1u << (x & 31u))) == 0;
@@ -320,10 +366,13 @@ constexpr int EVAL_INT = 1;
 // integer path is half as large
 constexpr int EVAL_INT128 = 3;
 
-template <int EF_CAP, int EVAL>
+// W > 1: wave 0 of a split-kernel workgroup, sharing every distance
+// evaluation with the workgroup's other W - 1 waves through sp.
+template <int EF_CAP, int EVAL, int W = 1>
 __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
                            const float* pmin, const float* pscale, const WaveLds& w,
-                           uint32_t cache_bits, uint32_t lane, uint32_t* vbits) {
+                           uint32_t cache_bits, uint32_t lane, uint32_t* vbits,
+                           SplitLds* sp = nullptr) {
     constexpr int S = EF_CAP / kWave;
     // clamps keep register, LDS and output indexing in bounds whatever the arrays hold
     const uint32_t ef =
@@ -386,8 +435,19 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
             return eval_fresh_int<8>(ix, qp, qq, w.bids, m, lane);
         else if constexpr (EVAL == EVAL_INT128)
             return eval_fresh_int<4>(ix, qp, qq, w.bids, m, lane);
-        else
+        else if constexpr (W > 1) {
+            if (lane == 0) {
+                sp->m = m;
+                sp->go = 1u;
+            }
+            __syncthreads();  // the other waves start their shares
+            eval_share<W, split_nb<EF_CAP>()>(ix, w.qv, pmin, pscale, w.bids, m, 0u, lane,
+                                              sp->keys);
+            __syncthreads();  // every share's keys are written
+            return lane < m ? sp->keys[lane] : ~0ull;
+        } else {
             return eval_fresh(ix, w.qv, pmin, pscale, w.bids, m, lane);
+        }
     };
     {
         const uint64_t k0 = readlane64(evaluate(1), 0);
@@ -804,6 +864,60 @@ __global__ void __launch_bounds__(128) h1_latency_kernel(DevIndex ix, SearchJob 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Split evaluation, for the SQ8 float path at high dimension: one query per
+// workgroup of W waves.  Wave 0 walks the query with search_one; at every
+// pass all W waves evaluate a share of the fresh candidates (each candidate
+// by the same group_l2 code as the batch kernels, so results are the same),
+// with blocks of split_nb chunks per row load.  At D = 960 a row is then two
+// memory round trips instead of eight, and a pass's arithmetic is spread over
+// W SIMDs.  The waves meet at two barriers per pass (SplitLds).
+// Registers: at most 128 VGPRs for W = 4, so that four waves per SIMD, a
+// 1,000-query launch's workgroups, are resident at once; W = 2 needs half as
+// many waves and is left at 256.
+template <int EF_CAP, int W>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W == 4 ? 4 : 2)))
+h1_split_kernel(DevIndex ix, SearchJob job,
+                                                       uint32_t cache_bits, uint32_t dim_pad,
+                                                       uint32_t wave_bytes) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wid = uniform(threadIdx.x / kWave);
+    float* pmin = reinterpret_cast<float*>(lds);
+    float* pscale = pmin + dim_pad;
+    for (uint32_t i = threadIdx.x; i < ix.dim; i += blockDim.x) {
+        pmin[i] = ix.sq8_min[i];
+        pscale[i] = ix.sq8_scale[i];
+    }
+    unsigned char* wb = lds + 8u * dim_pad;
+    WaveLds w;
+    w.qv = reinterpret_cast<float*>(wb);
+    w.stage = reinterpret_cast<uint64_t*>(wb + 4u * dim_pad);
+    w.bids = reinterpret_cast<uint32_t*>(w.stage + EF_CAP);
+    w.cache = w.bids + kWave;
+    SplitLds* sp = reinterpret_cast<SplitLds*>(wb + wave_bytes);
+    __syncthreads();
+    // a float launch after an integer launch that left nothing: done (the
+    // same value for every wave of the workgroup)
+    if (job.skip_int && job.left && uniform(*job.left) == 0u) return;
+    const uint32_t q = blockIdx.x;
+    if (wid == 0) {
+        if (q < job.nq)
+            search_one<EF_CAP, EVAL_FLOAT, W>(ix, job, q, pmin, pscale, w, cache_bits, lane,
+                                              nullptr, sp);
+        if (lane == 0) sp->go = 0u;
+        __syncthreads();
+    } else {
+        for (;;) {
+            __syncthreads();
+            if (uniform(sp->go) == 0u) break;
+            eval_share<W, split_nb<EF_CAP>()>(ix, w.qv, pmin, pscale, w.bids, uniform(sp->m),
+                                              wid, lane, sp->keys);
+            __syncthreads();
+        }
+    }
+}
+
 __global__ void reset_counter_kernel(uint32_t* counter, uint32_t n) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) counter[i] = 0u;
 }
@@ -857,6 +971,26 @@ LatencyFn select_latency(const LaunchCfg& cfg) {
                                 : latency_for_eval<EVAL_FLOAT>(cfg.ef_cap);
 }
 
+template <int W>
+KernelFn split_for_w(int ef_cap) {
+    switch (ef_cap) {
+        case 64: return &h1_split_kernel<64, W>;
+        case 128: return &h1_split_kernel<128, W>;
+        case 192: return &h1_split_kernel<192, W>;
+        case 256: return &h1_split_kernel<256, W>;
+        case 512: return &h1_split_kernel<512, W>;
+        case 1024: return &h1_split_kernel<1024, W>;
+        default: return nullptr;
+    }
+}
+
+KernelFn select_split(const LaunchCfg& cfg) {
+    if (cfg.eval != EVAL_FLOAT) return nullptr;
+    if (cfg.split == 2) return split_for_w<2>(cfg.ef_cap);
+    if (cfg.split == 4) return split_for_w<4>(cfg.ef_cap);
+    return nullptr;
+}
+
 KernelFn select_kernel(const LaunchCfg& cfg) {
     if (cfg.eval == EVAL_INT128) return select_for_eval<EVAL_INT128>(cfg.ef_cap, cfg.occupancy);
     return cfg.eval == EVAL_INT ? select_for_eval<EVAL_INT>(cfg.ef_cap, cfg.occupancy)
@@ -904,6 +1038,24 @@ int search_groups(const DevIndex& ix, const SearchJob& job, const LaunchCfg& cfg
 
 hipError_t launch_search(const DevIndex& ix, const SearchJob& job, const LaunchCfg& cfg,
                          hipStream_t stream) {
+    if (cfg.split > 1 && cfg.eval == EVAL_FLOAT && !job.visit_bits) {
+        // split evaluation: one query per workgroup of cfg.split waves
+        KernelFn k = select_split(cfg);
+        if (!k || cfg.cache_bits < 4 || cfg.cache_bits > 14) return hipErrorInvalidValue;
+        uint32_t dim_pad, wave_bytes;
+        geometry(ix, cfg, dim_pad, wave_bytes);
+        const size_t lds = 8u * static_cast<size_t>(dim_pad) + wave_bytes + sizeof(SplitLds);
+        if (lds > 64 * 1024) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               static_cast<int>(lds));
+            if (e != hipSuccess) return e;
+        }
+        if (job.nq == 0) return hipSuccess;
+        hipLaunchKernelGGL(k, dim3(job.nq), dim3(kWave * cfg.split), lds, stream, ix, job,
+                           static_cast<uint32_t>(cfg.cache_bits), dim_pad, wave_bytes);
+        return hipGetLastError();
+    }
     if (cfg.prefetch > 0 && job.nq <= kLatencyMaxQueries && !job.visit_bits) {
         // latency mode: one query per workgroup (search wave + prefetch wave)
         LatencyFn k = select_latency(cfg);

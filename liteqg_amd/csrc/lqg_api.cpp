@@ -118,7 +118,7 @@ int class_slot(int ef_cap) {
     return kClasses - 1;
 }
 
-LaunchCfg default_launch(int ef_cap) {
+LaunchCfg default_launch(int ef_cap, uint64_t dim) {
     LaunchCfg c;
     c.ef_cap = ef_cap;
     c.waves_per_group = 4;
@@ -130,6 +130,12 @@ LaunchCfg default_launch(int ef_cap) {
     // on the 1M x 128 workload at ef 50 a lone search() went from p50 0.115-
     // 0.120 ms to 0.100 ms, a 256-query batch from 0.216 to 0.186 ms
     c.prefetch = 2;
+    // SQ8 float path at D >= 512: two waves per query share each pass's
+    // distance evaluation (h1_split_kernel).  200k-node graphs, 1,000 queries,
+    // ef U{64..1024} in one launch: 960-d 26.2 -> 21.3 ms, 768-d 12.1 -> 9.6 ms
+    // (four waves: 23.1 and 10.6 ms); a lone 960-d query at ef 64 1.19 ms
+    // (latency mode) -> 0.92 ms.  At 96-d the batch was slower (5.7 -> 6.1 ms).
+    c.split = dim >= kSplitMinDim ? 2 : 1;
     return c;
 }
 
@@ -168,9 +174,15 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
     job.visit_waves = 0;
     job.done = sc->done;
     const int cls = ef_class(ef_max);
-    LaunchCfg cfg = h->launch_set[class_slot(cls)] ? h->launch[class_slot(cls)] : default_launch(cls);
+    LaunchCfg cfg = h->launch_set[class_slot(cls)] ? h->launch[class_slot(cls)]
+                                                   : default_launch(cls, h->dim);
     cfg.ef_cap = cls;
     if (nq < static_cast<uint64_t>(cfg.waves_per_group)) cfg.waves_per_group = 1;
+    // the default split applies to launches too small to fill the device
+    // with one wave per query; larger ones (a graph build's beam search of
+    // every node: 300k x 960, 8.3 s unsplit against 8.6 s split) run the
+    // batch kernel
+    if (!h->launch_set[class_slot(cls)] && nq > kSplitMaxQueries) cfg.split = 1;
     // An index whose SQ8 decode is exact (min 0, scale 1; D % 16 == 0, D <=
     // 256) runs the integer-path kernel first; a float-path launch then takes
     // only the queries that were not integer valued.  Each kernel variant
@@ -193,9 +205,11 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
         LQG_HIP_TRY(h->d_visit.reserve(words));
         job.visit_bits = h->d_visit.p;
     }
-    // a single latency-mode launch (one query per workgroup) uses no queue
+    // a single latency-mode or split launch (one query per workgroup) uses no queue
     const bool one_launch = !count_visits && (!int_index || queries_int == 0 || queries_int == 1);
-    if (!(one_launch && cfg.prefetch > 0 && nq <= kLatencyMaxQueries))
+    const bool float_only = !int_index || queries_int == 0;
+    if (!(one_launch && ((cfg.prefetch > 0 && nq <= kLatencyMaxQueries) ||
+                         (float_only && cfg.split > 1))))
         LQG_HIP_TRY(launch_reset_counter(sc->counter.p, s, static_cast<uint32_t>(nwords)));
     uint32_t* left = sc->counter.p + 2 * kQueueWords;
     // queries_int: the caller has checked on the host that every query is
@@ -580,7 +594,8 @@ lqg_status lqg_set_launch(lqg_index* h, uint32_t ef, const lqg_launch* cfg) {
     if (!h) return fail(LQG_ERR_ARG, "null index");
     if (ef != 0 && ef_class(ef) == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024");
     if (cfg && (cfg->waves_per_group < 1 || cfg->waves_per_group > 4 || cfg->cache_bits < 4 ||
-                cfg->cache_bits > 14 || cfg->groups < 0 || cfg->prefetch < 0 || cfg->prefetch > 3))
+                cfg->cache_bits > 14 || cfg->groups < 0 || cfg->prefetch < 0 || cfg->prefetch > 3 ||
+                cfg->split < 0 || cfg->split == 3 || cfg->split > 4))
         return fail(LQG_ERR_ARG, "launch settings out of range");
     const bool fs = h->mode == LQG_MODE_FASTSCAN;
     if (fs && cfg && cfg->cache_bits < 6)
@@ -623,6 +638,7 @@ lqg_status lqg_set_launch(lqg_index* h, uint32_t ef, const lqg_launch* cfg) {
         c.occupancy = cfg->occupancy;
         c.groups = cfg->groups;
         c.prefetch = cfg->prefetch;
+        c.split = cfg->split > 1 ? cfg->split : 1;
         h->launch[slot] = c;
         h->launch_set[slot] = true;
     }
@@ -641,14 +657,17 @@ lqg_status lqg_get_launch(const lqg_index* h, uint32_t ef, lqg_launch* cfg) {
         cfg->occupancy = c.occupancy;
         cfg->groups = c.groups;
         cfg->prefetch = c.prefetch;
+        cfg->split = 1;
         return LQG_OK;
     }
-    const lqg::LaunchCfg c = h->launch_set[slot] ? h->launch[slot] : lqg::default_launch(cls);
+    const lqg::LaunchCfg c =
+        h->launch_set[slot] ? h->launch[slot] : lqg::default_launch(cls, h->dim);
     cfg->waves_per_group = c.waves_per_group;
     cfg->cache_bits = c.cache_bits;
     cfg->occupancy = c.occupancy;
     cfg->groups = c.groups;
     cfg->prefetch = c.prefetch;
+    cfg->split = c.split;
     return LQG_OK;
 }
 
@@ -737,13 +756,14 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
             best->occupancy = best_cfg.occupancy;
             best->groups = best_cfg.groups;
             best->prefetch = best_cfg.prefetch;
+            best->split = 1;
         }
         return LQG_OK;
     }
     const bool had = h->launch_set[slot];
     const lqg::LaunchCfg saved = h->launch[slot];
     float best_ms = 1e30f;
-    lqg::LaunchCfg best_cfg = lqg::default_launch(cls);
+    lqg::LaunchCfg best_cfg = lqg::default_launch(cls, h->dim);
     // -1 (uncapped) differs from the default only up to the 256 class
     const int occs[] = {0, 6, 8, -1};
     const int n_occ = cls <= 256 ? 4 : 3;
@@ -753,16 +773,28 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
     // only, and keeps the class's prefetch depth for small launches (a lone
     // search() after tuning on a large batch still runs in latency mode).
     const bool small = nq <= lqg::kLatencyMaxQueries;
-    const int keep_pf = had ? saved.prefetch : lqg::default_launch(cls).prefetch;
+    const int keep_pf = had ? saved.prefetch : lqg::default_launch(cls, h->dim).prefetch;
     std::vector<lqg::LaunchCfg> cands;
     for (int oi = 0; oi < n_occ; ++oi)
         for (int cb : bits)
             for (int wg : wpg) {
-                lqg::LaunchCfg c = lqg::default_launch(cls);
+                lqg::LaunchCfg c = lqg::default_launch(cls, h->dim);
                 c.occupancy = occs[oi];
                 c.cache_bits = cb;
                 c.waves_per_group = wg;
                 c.prefetch = small ? 0 : keep_pf;
+                c.split = 1;
+                cands.push_back(c);
+            }
+    // split evaluation (the float path only: an index without an exact
+    // integer decode), 2 or 4 waves per query
+    const bool int_index = h->sq8_integer && h->sq8_identity && h->dim % 16 == 0 && h->dim <= 256;
+    if (!int_index)
+        for (int sw : {2, 4})
+            for (int cb : bits) {
+                lqg::LaunchCfg c = lqg::default_launch(cls, h->dim);
+                c.cache_bits = cb;
+                c.split = sw;
                 cands.push_back(c);
             }
     // a tuning set as small as a latency-mode launch also tries that mode at
@@ -771,9 +803,10 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
     if (small)
         for (int pf : {1, 2, 3})
             for (int cb : bits) {
-                lqg::LaunchCfg c = lqg::default_launch(cls);
+                lqg::LaunchCfg c = lqg::default_launch(cls, h->dim);
                 c.cache_bits = cb;
                 c.prefetch = pf;
+                c.split = 1;
                 cands.push_back(c);
             }
     for (const lqg::LaunchCfg& c : cands) {
@@ -813,6 +846,7 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
         best->occupancy = best_cfg.occupancy;
         best->groups = best_cfg.groups;
         best->prefetch = best_cfg.prefetch;
+        best->split = best_cfg.split;
     }
     return LQG_OK;
 }

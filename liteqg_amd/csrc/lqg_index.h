@@ -82,8 +82,16 @@ int ef_class(uint32_t ef);
 int class_slot(int ef_cap);
 int class_cap(int slot);
 
-// Launch settings used when none were set for a class.
-LaunchCfg default_launch(int ef_cap);
+// Launch settings used when none were set for a class, on an index of
+// dimension dim.
+LaunchCfg default_launch(int ef_cap, uint64_t dim);
+
+// Smallest dimension whose SQ8 float-path launches split each query's
+// distance evaluation over two waves by default.
+constexpr uint64_t kSplitMinDim = 512;
+// Largest launch the default split applies to (four waves per SIMD of the
+// device with one wave per query).
+constexpr uint64_t kSplitMaxQueries = 4096;
 
 }  // namespace lqg
 

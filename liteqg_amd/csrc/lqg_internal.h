@@ -85,6 +85,8 @@ struct LaunchCfg {
                          // 3: the same for rows of at most 128 bytes
     int prefetch = 0;    // launches of at most kLatencyMaxQueries queries: 0 = the batch
                          // kernel; 1..3 = latency mode with that prefetch depth
+    int split = 1;       // SQ8 float path: waves per query sharing each pass's
+                         // distance evaluation (h1_split_kernel); 1 = off, 2 or 4
 };
 
 // Largest launch that takes the latency-mode kernel (one query per

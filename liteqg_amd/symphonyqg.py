@@ -151,11 +151,14 @@ class Index:
         return out[0] if len(out) == 1 else tuple(out)
 
     def set_launch(self, ef: int = 0, waves_per_group: int = 4, cache_bits: int = 10,
-                   occupancy: int = 0, groups: int = 0, prefetch: int = 0) -> None:
+                   occupancy: int = 0, groups: int = 0, prefetch: int = 0,
+                   split: int = 1) -> None:
         """Kernel launch settings for ef's capacity class (ef=0: every class).
         prefetch (SQ8 launches of at most 256 queries, e.g. a lone search()):
-        0 = the batch kernel, 1..3 = latency mode with that prefetch depth."""
-        cfg = Launch(waves_per_group, cache_bits, occupancy, groups, prefetch)
+        0 = the batch kernel, 1..3 = latency mode with that prefetch depth.
+        split (SQ8 float path): 1 = off, 2 or 4 waves per query sharing each
+        expansion's distance evaluation (takes precedence over prefetch)."""
+        cfg = Launch(waves_per_group, cache_bits, occupancy, groups, prefetch, split)
         check(lib().lqg_set_launch(self._h, int(ef), C.byref(cfg)))
 
     def reset_launch(self, ef: int = 0) -> None:

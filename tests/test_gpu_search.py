@@ -150,6 +150,66 @@ def test_latency_mode_matches_oracle(oracle_mod, tmp_path, prefetch, kind, d, de
     assert np.array_equal(np.stack([ix.search(q, 10) for q in Q[:20]]), exp)
 
 
+@pytest.mark.parametrize("split", [2, 4])
+@pytest.mark.parametrize("kind,d,deg", [("int", 128, 32), ("uniform", 96, 64),
+                                        ("uniform", 768, 32), ("uniform", 960, 64),
+                                        ("uniform", 100, 32)])
+def test_split_evaluation_matches_oracle(oracle_mod, tmp_path, split, kind, d, deg):
+    """Split evaluation (one query per workgroup of 2 or 4 waves sharing each
+    pass's distance evaluation): ids and distance bits equal the oracle's in
+    both orders at ef 1, 50, 150 and 600, with per-query ef and k in one
+    launch, and for the plain search.  The integer index runs it with its
+    integer kernel disabled; D = 100 takes the generic row loop."""
+    n = 3000 if d <= 128 else 1500
+    arr, base = make_random_index(n, d, deg, seed=d + split, kind=kind, knn=True)
+    ix = device_from_arrays(arr, tmp_path)
+    if kind == "int":
+        ix.set_sq8_integer(False)
+    ix.set_launch(0, waves_per_group=1, cache_bits=10, split=split)
+    assert ix.get_launch(300)["split"] == split
+    rng = np.random.default_rng(d)
+    nq = 120
+    Q = (base[rng.integers(0, n, nq)] + rng.normal(0, 0.05 if kind == "uniform" else 10,
+                                                   size=(nq, d))).astype(np.float32)
+    if kind == "int":
+        Q = np.clip(np.rint(Q), 0, 255).astype(np.float32)
+    for order in ("reference", "nearest"):
+        for ef in (1, 50, 150, 600):
+            for m in (1, 37, nq):
+                ids, dis = ix.search_batch(Q[:m], 10, ef=ef, order=order, return_distances=True)
+                exp, exp_d, _ = oracle_batch(oracle_mod, arr, Q[:m], ef, 10, order)
+                assert_same(ids, dis, exp, exp_d)
+    efs = rng.integers(1, 1025, nq).astype(np.uint32)
+    ks = rng.choice([1, 10, 100], nq).astype(np.uint32)
+    ids, dis = ix.search_batch(Q, ks, ef=efs, order="nearest", return_distances=True)
+    for i in range(nq):
+        e_ids, e_d, _ = oracle_batch(oracle_mod, arr, Q[i:i + 1], int(efs[i]), int(ks[i]),
+                                     "nearest")
+        kk = int(ks[i])
+        assert np.array_equal(ids[i, :kk], e_ids[0]), i
+        assert np.array_equal(dis[i, :kk].view(np.uint32), e_d[0].view(np.uint32)), i
+        assert (ids[i, kk:] == 0xFFFFFFFF).all()
+    ix.set_ef(150)
+    ix.set_order("nearest")
+    exp, _, _ = oracle_batch(oracle_mod, arr, Q[:10], 150, 10, "nearest")
+    assert np.array_equal(np.stack([ix.search(q, 10) for q in Q[:10]]), exp)
+
+
+def test_split_is_default_at_high_dimension(oracle_mod, tmp_path):
+    """An index of D >= 512 splits its float-path evaluation over two waves
+    by default; autotuning on it considers split launches and keeps results."""
+    arr, base = make_random_index(1500, 512, 32, seed=3, knn=True)
+    ix = device_from_arrays(arr, tmp_path)
+    assert ix.get_launch(50)["split"] == 2
+    rng = np.random.default_rng(4)
+    Q = (base[rng.integers(0, 1500, 64)] + rng.normal(0, 0.05, (64, 512))).astype(np.float32)
+    best = ix.autotune(Q, 50, 10)
+    assert best["split"] in (1, 2, 4)
+    exp, exp_d, _ = oracle_batch(oracle_mod, arr, Q, 50, 10, "nearest")
+    ids, dis = ix.search_batch(Q, 10, ef=50, order="nearest", return_distances=True)
+    assert_same(ids, dis, exp, exp_d)
+
+
 def test_autotune_and_prefetch_depth(oracle_mod, tmp_path):
     """The default prefetch depth of small launches survives tuning on a large
     batch (which times the batch kernel only); tuning on a small set picks a
