@@ -262,17 +262,31 @@ __device__ __forceinline__ float exact_l2_regs(const float (&xr)[kRegJ], const f
         const float t = qr[u] - xr[u];
         s = fmaf(t, t, s);
     }
-    // dims past the registers' (D > 16 kRegJ), 16 per partial at a time: a
-    // block's loads are issued together, then its FMAs run in dim order
-    for (uint32_t j0 = kRegJ; j0 < nj; j0 += 16u) {
+    // Dims past the registers' (D > 16 kRegJ), in blocks of 16 per partial.
+    // Up to four blocks per round trip: lane row h (lanes 16h .. 16h + 15)
+    // loads block h of the round, so all four are in flight together; block
+    // b then reaches lane p from lane p + 16b (ds_bpermute), and the FMAs run
+    // in dim order.  Steps past the chain (a last block's tail) take t = 0:
+    // fma(0, 0, s) = s.
+    const uint32_t h = lane >> 4;
+    for (uint32_t j0 = kRegJ; j0 < nj; j0 += 64u) {
+        const uint32_t jb = j0 + 16u * h;
         float xv[16];
 #pragma unroll
-        for (uint32_t u = 0; u < 16u; ++u) xv[u] = j0 + u < nj ? row[p + 16u * (j0 + u)] : 0.0f;
+        for (uint32_t u = 0; u < 16u; ++u) xv[u] = jb + u < nj ? row[p + 16u * (jb + u)] : 0.0f;
 #pragma unroll
-        for (uint32_t u = 0; u < 16u; ++u) {
-            if (j0 + u < nj) {
-                const float t = w.qv[p + 16u * (j0 + u)] - xv[u];
-                s = fmaf(t, t, s);
+        for (uint32_t b = 0; b < 4u; ++b) {
+            const uint32_t jj = j0 + 16u * b;
+            if (jj >= nj) break;
+            const int src = static_cast<int>(4u * (p + 16u * b));
+#pragma unroll
+            for (uint32_t u = 0; u < 16u; ++u) {
+                const float x = b == 0 ? xv[u]
+                                       : __int_as_float(__builtin_amdgcn_ds_bpermute(
+                                             src, __float_as_int(xv[u])));
+                const bool ok = jj + u < nj;
+                const float t = w.qv[p + 16u * min(jj + u, nj - 1u)] - x;
+                s = fmaf(ok ? t : 0.0f, ok ? t : 0.0f, s);
             }
         }
     }
