@@ -264,7 +264,7 @@ __device__ bool h2_fast_one(const H2Dev& ix, const H2Job& job, uint32_t q, const
     const LutBase lutb = lut_base<CW>(w.lut, Sh::kP, part);
     const float* qsrc = job.queries + static_cast<size_t>(q) * Sh::kDim;
     Prep pr = prepare(ix, qsrc, job.rot ? job.rot + static_cast<size_t>(q) * Sh::kP : nullptr, w,
-                      2u, lane);
+                      2u, lane, false);
     pr.lo = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pr.lo)));
     pr.width = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(pr.width)));
     pr.sumq = __builtin_amdgcn_readfirstlane(pr.sumq);

@@ -46,7 +46,7 @@
 namespace lqg {
 
 struct Layout {
-    uint32_t qv, tb, qb, lut, st, hv, wave_bytes;
+    uint32_t qv, tb, qb, lut, pl, st, hv, wave_bytes;
     int has_qv;
 };
 
@@ -62,6 +62,7 @@ __host__ __device__ inline Layout h2_layout(uint32_t dim, uint32_t p, uint32_t e
     l.has_qv = with_qv ? 1 : 0;
     l.qv = o;   o += with_qv ? align16(4 * dim) : 0u;
     l.lut = o;  o += align16(2u * p + 256u * ((p / 8u + 15u) / 16u));
+    l.pl = o;   o += align16(p);  // bit planes: 8 words per 32-bit code word
     // one region for the preparation buffers (tb, qb) and, once the LUT is
     // built, the pool merge staging (ef_cap records and a spare)
     const uint32_t prep = align16(4 * p) + align16(p);
@@ -100,6 +101,7 @@ struct H2Lds {
     float* tb;        // rotated query during preparation; q - x in exact_l2 (p floats)
     uint8_t* qb;      // quantised query (p bytes)
     uint8_t* lut;     // 4p bytes (+16 bytes of bank skew for 2 lanes per vector)
+    uint4* pl;        // query bit planes (see prepare): 2 uint4 per 32-bit code word
     uint4* st;        // pool merge staging, once the LUT is built (shares tb's memory):
                       // EF_CAP + 1 records {key lo, key hi, id, 0}
     uint32_t* hv;     // visited table (1 << hash_bits)
@@ -111,6 +113,7 @@ __device__ __forceinline__ H2Lds carve(unsigned char* base, const Layout& l) {
     w.tb = reinterpret_cast<float*>(base + l.tb);
     w.qb = base + l.qb;
     w.lut = base + l.lut;
+    w.pl = reinterpret_cast<uint4*>(base + l.pl);
     w.st = reinterpret_cast<uint4*>(base + l.st);
     w.hv = reinterpret_cast<uint32_t*>(base + l.hv);
     return w;
@@ -136,6 +139,8 @@ __host__ __device__ inline uint32_t lut_bytes(uint32_t p) {
 struct Prep {
     float lo, width;
     int32_t sumq;
+    // 1: the bit-plane sum (plane_sum) equals the LUT scan for this query
+    int planes;
 };
 
 // In-place unnormalised Walsh-Hadamard transform of t[0..p) in LDS by one
@@ -156,7 +161,7 @@ __device__ void fht_lds(float* t, uint32_t p, uint32_t lane) {
 // rsrc: the query already rotated (p floats, the GEMM rotation), or null to
 // rotate here with the FHT.
 __device__ Prep prepare(const H2Dev& ix, const float* qsrc, const float* rsrc, const H2Lds& w,
-                        uint32_t lpv, uint32_t lane) {
+                        uint32_t lpv, uint32_t lane, bool want_planes = true) {
     const uint32_t dim = ix.dim, p = ix.p;
     if (w.qv)
         for (uint32_t i = lane; i < dim; i += kWave) w.qv[i] = qsrc[i];
@@ -210,12 +215,52 @@ __device__ Prep prepare(const H2Dev& ix, const float* qsrc, const float* rsrc, c
             if ((j >> b) & 1u) s += w.qb[4 * c + 3 - b];
         w.lut[lut_pos(p, c, j)] = static_cast<uint8_t>(s & 0xFFu);
     }
-    wave_sync();
-    Prep r;
-    r.lo = lo;
-    r.width = width;
-    r.sumq = sq;
-    return r;
+    // Bit planes.  Code bit i of 32-bit code word W (little-endian bytes
+    // 4W .. 4W + 3) stands for dim d(W, i) = 8B + r, B = 4W + i / 8, with
+    // r = 3 - m for bit m = i % 8 below 4 (codebook 2B, bit 3 = dim 8B) and
+    // r = 11 - m above (codebook 2B + 1).  Plane k of word W holds bit k of
+    // q[d(W, i)] at bit i, at w.pl word 8W + k (k < 7; q <= 64).  Then the
+    // scan's sum, the LUT entries of a code added up, is
+    //   sum_k 2^k popcount(code word AND plane k word), over the words,
+    // as long as no LUT entry wrapped: an entry is the uint8 sum of up to
+    // four q, so only a codebook whose four q add up to 256 or more (all four
+    // 64) differs; such a query keeps the LUT scan.  Lane l takes code bit l
+    // of the word pair (2t, 2t + 1): dim 64t + 8 (l / 8) + r(l % 8); one ballot
+    // per plane gives the pair's two words.
+    int planes = 0;
+    if (want_planes) {
+        bool wrap = false;
+        const uint32_t m = lane & 7u;
+        const uint32_t r = m < 4u ? 3u - m : 11u - m;
+        for (uint32_t t = 0; 64u * t < p; ++t) {
+            const uint32_t q = w.qb[64u * t + (lane & ~7u) + r];
+            // the codebook of this bit: lanes 8g .. 8g + 3 or 8g + 4 .. 8g + 7
+            uint32_t cs = q + static_cast<uint32_t>(__shfl_xor(static_cast<int>(q), 1));
+            cs += static_cast<uint32_t>(__shfl_xor(static_cast<int>(cs), 2));
+            wrap = wrap || cs > 255u || q > 127u;
+            uint32_t* pw = reinterpret_cast<uint32_t*>(w.pl) + 16u * t;
+#pragma unroll
+            for (uint32_t k = 0; k < 7u; ++k) {
+                const uint64_t b = __ballot((q >> k) & 1u);
+                if (lane == 0) {
+                    pw[k] = static_cast<uint32_t>(b);
+                    pw[8u + k] = static_cast<uint32_t>(b >> 32);
+                }
+            }
+            if (lane == 0) {
+                pw[7] = 0u;
+                pw[15] = 0u;
+            }
+        }
+        planes = __ballot(wrap) == 0 ? 1 : 0;
+        wave_sync();
+    }
+    Prep res;
+    res.lo = lo;
+    res.width = width;
+    res.sumq = sq;
+    res.planes = planes;
+    return res;
 }
 
 // The same distance with the partials in registers: lane p < 16 runs the FMA
@@ -557,14 +602,35 @@ __device__ __forceinline__ void load_codes(uint32_t (&cw)[CW], gptr<uint32_t> ba
     }
 }
 
+// The lane's FastScan sum by the query's bit planes (prepare): per code
+// word, seven ANDs with the plane words (one uniform LDS address per
+// instruction for one lane per neighbour, two for two) and seven population
+// counts, weighted 2^k.  pl: the plane words of the lane's first code word.
+template <int CW>
+__device__ __forceinline__ uint32_t plane_sum(const uint32_t (&cw)[CW], const uint4* pl) {
+    uint32_t a[7] = {0u, 0u, 0u, 0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int k = 0; k < CW; ++k) {
+        const uint4 x = pl[2 * k], y = pl[2 * k + 1];
+        a[0] += static_cast<uint32_t>(__builtin_popcount(cw[k] & x.x));
+        a[1] += static_cast<uint32_t>(__builtin_popcount(cw[k] & x.y));
+        a[2] += static_cast<uint32_t>(__builtin_popcount(cw[k] & x.z));
+        a[3] += static_cast<uint32_t>(__builtin_popcount(cw[k] & x.w));
+        a[4] += static_cast<uint32_t>(__builtin_popcount(cw[k] & y.x));
+        a[5] += static_cast<uint32_t>(__builtin_popcount(cw[k] & y.y));
+        a[6] += static_cast<uint32_t>(__builtin_popcount(cw[k] & y.z));
+    }
+    return a[0] + 2u * a[1] + 4u * a[2] + 8u * a[3] + 16u * a[4] + 32u * a[5] + 64u * a[6];
+}
+
 // Estimates of the neighbours a lane holds for one node (pass of VP
 // neighbours starting at vb).  Returns the estimate on every lane of the
-// neighbour's lanes.
+// neighbour's lanes.  pl: the lane's plane words (used when pr.planes).
 template <int CW>
 __device__ __forceinline__ float lane_estimate(const uint32_t (&cw)[CW], LutBase lb,
-                                               uint32_t lpv, const Prep& pr, float sqr_y,
-                                               float tx, float dq, float vq) {
-    uint32_t s = scan_sum<CW>(cw, lb);
+                                               const uint4* pl, uint32_t lpv, const Prep& pr,
+                                               float sqr_y, float tx, float dq, float vq) {
+    uint32_t s = pr.planes ? plane_sum<CW>(cw, pl) : scan_sum<CW>(cw, lb);
     // the two lanes of a neighbour: DPP quad_perm [1, 0, 3, 2]
     if (lpv == 2)
         s += static_cast<uint32_t>(
@@ -606,6 +672,7 @@ __device__ bool h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
     const uint32_t code_bytes = ix.p / 8;         // per neighbour
     const uint32_t part = lane % lpv;
     const LutBase lutb = lut_base<CW>(w.lut, ix.p, part);
+    const uint4* plv = w.pl + 2u * CW * part;
 
     const Prep pr = prepare(ix, job.queries + static_cast<size_t>(q) * dim,
                             job.rot ? job.rot + static_cast<size_t>(q) * ix.p : nullptr, w, lpv,
@@ -710,7 +777,7 @@ __device__ bool h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
             const float vq = uniform_gptr<float>(fac + 2 * deg + vb)[lv];
             // scan_neighbors (qg.hpp:355): exact distance to the node
             if (vb == 0) sqr_y = exact_l2_regs(xr, qr, raw, w, dim, lane);
-            const float est = lane_estimate<CW>(cw, lutb, lpv, pr, sqr_y, tx, dq, vq);
+            const float est = lane_estimate<CW>(cw, lutb, plv, lpv, pr, sqr_y, tx, dq, vq);
             // is_full / visited_.get (qg.hpp:375-377); candidate i gets insertion
             // number seq + i.  The visited table is probed only by the lanes the
             // pool would take.

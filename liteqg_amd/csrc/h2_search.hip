@@ -97,7 +97,8 @@ __global__ void __launch_bounds__(64) h2_probe_kernel(H2Dev ix, const float* que
 #pragma unroll
             for (int k = 0; k < CW; ++k) cw[k] = 0;
         }
-        const float est = lane_estimate<CW>(cw, lutb, lpv, pr, sqr_y, tx, dq, vq);
+        const float est = lane_estimate<CW>(cw, lutb, w.pl + 2u * CW * part, lpv, pr, sqr_y, tx,
+                                            dq, vq);
         if (act && part == 0) est_out[static_cast<size_t>(q) * deg + vi] = est;
     }
 }

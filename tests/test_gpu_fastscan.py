@@ -225,6 +225,66 @@ def test_search_matches_oracle(oracle_mod, tmp_path, shape):
         assert st["evaluations"] == int(e_st[:, 1].sum())
 
 
+@pytest.mark.parametrize("n,d,deg", [(1500, 128, 32), (1500, 100, 32), (800, 256, 64),
+                                     (500, 768, 32), (400, 960, 64)])
+def test_bit_plane_scan_and_lut_wrap_match_oracle(oracle_mod, tmp_path, n, d, deg):
+    """The general kernel sums a neighbour's LUT entries by the query's bit
+    planes, except for a query with a codebook whose four quantised values
+    add up to 256 or more: its uint8 LUT entry wraps, and the query keeps
+    the LUT scan.  Scaled unit vectors c e_j rotate to vectors whose entries
+    are all +-c s_j, so half or all of them quantise to 64 (or 63): they
+    reach the wrap.  Estimates (probe kernel) and search results equal the
+    oracle's for queries on both sides; D = 128 forces the general kernel
+    over the specialised one."""
+    O = oracle_mod
+    arr, _ = make_random_index(n, d, deg, seed=d + 7, knn=True)
+    ix = _index_from_arrays(arr, tmp_path, "w")
+    p = ix.padded_dim
+    signs = _signs(p, d + 1)
+    ix.build_sidecar(signs=signs)
+    ix.set_generic_kernel(True)
+    ix.set_mode("fastscan")
+    rng = np.random.default_rng(d + 2)
+    def wraps(q):
+        # the single-bit entries are the codebook's q values (exact); the
+        # all-bits entry is their uint8 sum
+        lut = O.query_prepare(signs, q, p)[5].reshape(p // 4, 16).astype(np.int64)
+        return bool(((lut[:, 1] + lut[:, 2] + lut[:, 4] + lut[:, 8]) >= 256).any())
+
+    # columns 0, 4 and 64 of the Hadamard matrix have runs of equal signs at
+    # least a codebook long; whether the top entries quantise to 64 depends on
+    # the scale's rounding, so scales are tried until some queries wrap
+    Q = [(arr.raw[rng.integers(0, n, 20)] + rng.normal(0, 0.05, (20, d))).astype(np.float32)]
+    picked = {True: 0, False: 0}
+    for j in (0, 4, min(64, d - 1)):
+        for c in np.geomspace(0.1, 10, 40).astype(np.float32):
+            for sg in (1.0, -1.0):
+                e = np.zeros((1, d), np.float32)
+                e[0, j] = sg * c
+                w = wraps(e[0])
+                if picked[w] < 8:
+                    picked[w] += 1
+                    Q.append(e)
+    assert picked[True] >= 4 and picked[False] >= 4
+    Q = np.concatenate(Q)
+    wrap = [wraps(q) for q in Q]
+    nodes = rng.integers(0, n, Q.shape[0]).astype(np.uint32)
+    pr = ix.fastscan_probe(Q, nodes)
+    _, codes, fac = ix.get_sidecar()
+    H = O.H2Arrays(arr, signs, codes, fac, 1)
+    for i, q in enumerate(Q):
+        _, lo, width, sumq, _, lut = O.query_prepare(signs, q, p)
+        node = int(nodes[i])
+        _, est = O.scan_estimates(codes[node], 1, fac[node], deg, p, lut, float(pr["sqr_y"][i]),
+                                  lo, width, sumq)
+        assert np.array_equal(_bits(pr["est"][i]), _bits(est)), (i, wrap[i])
+    for ef, k in ((32, 10), (200, 20)):
+        ids, dis = ix.search_batch(Q, k, ef=ef, return_distances=True)
+        e_ids, e_dis, _ = O.search_h2_batch(H, Q, ef, k)
+        assert np.array_equal(ids, e_ids), f"ef={ef}"
+        assert np.array_equal(_bits(dis), _bits(e_dis)), f"ef={ef}"
+
+
 @pytest.mark.parametrize("shape", [s for s in SHAPES if s[1] <= 256],
                          ids=lambda s: f"n{s[0]}_d{s[1]}_deg{s[2]}")
 def test_occupancy_variants_match_oracle(oracle_mod, tmp_path, shape):
