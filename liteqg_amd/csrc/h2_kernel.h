@@ -674,9 +674,12 @@ __device__ bool h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
     const LutBase lutb = lut_base<CW>(w.lut, ix.p, part);
     const uint4* plv = w.pl + 2u * CW * part;
 
+    // bit planes from P = 256 on: at P = 128 the specialised kernel ran
+    // slower with them (0.559 -> 0.574 ms per 10,000-query launch), its LUT
+    // scan being 16 lookups per lane
     const Prep pr = prepare(ix, job.queries + static_cast<size_t>(q) * dim,
                             job.rot ? job.rot + static_cast<size_t>(q) * ix.p : nullptr, w, lpv,
-                            lane);
+                            lane, ix.p >= 256u);
 
     Visited<GLOBAL> vis;
     vis.tab = GLOBAL ? gtab : w.hv;

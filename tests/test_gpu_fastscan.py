@@ -226,12 +226,12 @@ def test_search_matches_oracle(oracle_mod, tmp_path, shape):
 
 
 @pytest.mark.parametrize("n,d,deg", [(1500, 128, 32), (1500, 100, 32), (800, 256, 64),
-                                     (500, 768, 32), (400, 960, 64)])
+                                     (800, 200, 32), (500, 768, 32), (400, 960, 64)])
 def test_bit_plane_scan_and_lut_wrap_match_oracle(oracle_mod, tmp_path, n, d, deg):
-    """The general kernel sums a neighbour's LUT entries by the query's bit
-    planes, except for a query with a codebook whose four quantised values
-    add up to 256 or more: its uint8 LUT entry wraps, and the query keeps
-    the LUT scan.  Scaled unit vectors c e_j rotate to vectors whose entries
+    """From P = 256 on, the general kernel sums a neighbour's LUT entries by
+    the query's bit planes, except for a query with a codebook whose four
+    quantised values add up to 256 or more: its uint8 LUT entry wraps, and
+    the query keeps the LUT scan (as every query at P <= 128 does).  Scaled unit vectors c e_j rotate to vectors whose entries
     are all +-c s_j, so half or all of them quantise to 64 (or 63): they
     reach the wrap.  Estimates (probe kernel) and search results equal the
     oracle's for queries on both sides; D = 128 forces the general kernel
