@@ -273,9 +273,12 @@ typedef struct {
                                 ahead what the next expansions read.  SQ8 -- 1: the
                                 id rows of the two best unexpanded pool entries,
                                 2: also their neighbours' SQ8 rows, 3: both for
-                                the four best.  FastScan: the node records of the
-                                2^prefetch best.  Results are the same; only time
-                                changes.  Default 2                                 */
+                                the four best, 4: the second wave computes the
+                                keys of all neighbours of the best unexpanded
+                                entry ahead of its expansion (speculative
+                                evaluation).  FastScan (0..3): the node records
+                                of the 2^prefetch best.  Results are the same;
+                                only time changes.  Default 2                     */
     int32_t split;           /* SQ8 float path (an index whose SQ8 decode is not the
                                 identity, or D > 256): waves per query sharing each
                                 expansion's distance evaluation, one query per

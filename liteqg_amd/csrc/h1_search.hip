@@ -349,6 +349,31 @@ __device__ __forceinline__ void eval_share(const DevIndex& ix, const float* qv, 
     if ((lane & ((1u << lg) - 1u)) == 0 && (lane >> lg) < mw) keys[c0 + (lane >> lg)] = key;
 }
 
+// Speculative evaluation (latency mode, depth 4): the second wave of a
+// query's workgroup computes the keys of every neighbour of the best
+// unexpanded pool entries before they are expanded, into these slots; wave 0
+// takes a popped node's ids and keys from its slot instead of loading and
+// evaluating them.  A slot's tag is the node it holds, kInvalid while the
+// slot is written (a seqlock: wave 0 reads the tag, the row, then the tag
+// again, and uses the row only if both reads give the node).
+constexpr uint32_t kSpecSlots = 4;
+
+struct SpecSlot {
+    uint32_t tag;
+    uint32_t pad[3];
+    uint32_t ids[kWave];   // the node's neighbour ids (degree <= 64)
+    uint64_t keys[kWave];  // make_key(distance, id) of each
+};
+
+struct SpecLds {
+    uint32_t qready;  // 1 once wave 0 has prepared the query
+    uint32_t qq;      // the integer path's sum of squares of the query
+    uint32_t cur;     // the node wave 0 expands
+    uint32_t pad;
+    uint32_t bids[kWave];  // the speculating wave's candidate list
+    SpecSlot slot[kSpecSlots];
+};
+
 // Marks node x in the counting bitmap; true when it was not marked before.
 __device__ __forceinline__ bool first_visit(uint32_t* bits, uint32_t x) {
     return (atomicOr(&bits[x >> 5], 1u << (x & 31u)) & (1u << (x & 31u))) == 0;
@@ -368,11 +393,13 @@ constexpr int EVAL_INT128 = 3;
 
 // W > 1: wave 0 of a split-kernel workgroup, sharing every distance
 // evaluation with the workgroup's other W - 1 waves through sp.
-template <int EF_CAP, int EVAL, int W = 1>
+// SPEC: wave 0 of a speculative latency-mode workgroup (h1_spec_kernel),
+// taking rows the other wave evaluated ahead from spec's slots.
+template <int EF_CAP, int EVAL, int W = 1, bool SPEC = false>
 __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
                            const float* pmin, const float* pscale, const WaveLds& w,
                            uint32_t cache_bits, uint32_t lane, uint32_t* vbits,
-                           SplitLds* sp = nullptr) {
+                           SplitLds* sp = nullptr, SpecLds* spec = nullptr) {
     constexpr int S = EF_CAP / kWave;
     // clamps keep register, LDS and output indexing in bounds whatever the arrays hold
     const uint32_t ef =
@@ -412,6 +439,14 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
         if (lane < ix.dim / 4) w.qv[lane] = __uint_as_float(word);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) qq += __shfl_xor(qq, o);
+    }
+    if constexpr (SPEC) {
+        // the query (packed in place for the integer path) is complete
+        wave_sync();
+        if (lane == 0) {
+            spec->qq = qq;
+            __hip_atomic_store(&spec->qready, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
     }
     const uint32_t csize = 1u << cache_bits;
     for (uint32_t i = lane; i < csize; i += kWave) w.cache[i] = kInvalid;
@@ -479,13 +514,42 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
         if (!found) break;
         ++n_exp;
         const uint32_t* nrow = ix.nbr + static_cast<size_t>(key_id(kc)) * ix.deg;
+        // the node's row from a speculation slot: lane l < 32 holds neighbours
+        // l (s_id0, s_k0) and 32 + l (s_id1, s_k1)
+        bool use_spec = false;
+        uint32_t s_id0 = kInvalid, s_id1 = kInvalid;
+        uint64_t s_k0 = ~0ull, s_k1 = ~0ull;
+        if constexpr (SPEC) {
+            const uint32_t x = key_id(kc);
+            if (lane == 0) __hip_atomic_store(&spec->cur, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            uint32_t t = kInvalid;
+            if (lane < kSpecSlots)
+                t = __hip_atomic_load(&spec->slot[lane].tag, __ATOMIC_ACQUIRE,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+            const uint64_t hit = __ballot(lane < kSpecSlots && t == x);
+            if (hit && ix.deg <= kWave) {
+                SpecSlot* sl = &spec->slot[__builtin_ctzll(hit)];
+                if (lane < 32) {
+                    s_id0 = sl->ids[lane];
+                    s_k0 = sl->keys[lane];
+                    if (32u + lane < ix.deg) {
+                        s_id1 = sl->ids[32u + lane];
+                        s_k1 = sl->keys[32u + lane];
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                const uint32_t t2 =
+                    __hip_atomic_load(&sl->tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                use_spec = uniform(t2) == x;
+            }
+        }
 
         for (uint32_t cb = 0; cb < ix.deg; cb += 32) {
             // neighbours not seen before (qg.hpp:320-324), 32 ids at a time
             bool fresh = false;
             uint32_t nid = kInvalid;
             if (lane < 32 && cb + lane < ix.deg) {
-                nid = nrow[cb + lane];
+                nid = use_spec ? (cb == 0 ? s_id0 : s_id1) : nrow[cb + lane];
                 if (nid < ix.n) {
                     const uint32_t slot = (nid * 2654435761u) >> (32 - cache_bits);
                     fresh = atomicExch(&w.cache[slot], nid) != nid;
@@ -494,7 +558,7 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
             const uint64_t fm = __ballot(fresh);
             if (fm == 0) continue;
             const uint32_t m = static_cast<uint32_t>(__popcll(fm));
-            if (fresh) w.bids[lanes_below(fm)] = nid;
+            if (!use_spec && fresh) w.bids[lanes_below(fm)] = nid;
             wave_sync();
             if (vbits) {
                 const bool first = fresh && first_visit(vbits, nid);
@@ -502,7 +566,8 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
             } else {
                 n_eval += m;
             }
-            const uint64_t key = evaluate(m);
+            const uint64_t key =
+                use_spec ? (fresh ? (cb == 0 ? s_k0 : s_k1) : ~0ull) : evaluate(m);
 
             // admission (qg.hpp:329-335): pool not full, or below its maximum
             uint64_t last = 0;
@@ -918,6 +983,126 @@ h1_split_kernel(DevIndex ix, SearchJob job,
     }
 }
 
+// Latency mode at depth 4: wave 1 computes, for the best unexpanded pool
+// entry that is neither the node wave 0 expands nor held by a slot, the keys
+// of all of its neighbours (the batch kernels' evaluation code, so the same
+// bits), and publishes them in the next slot (round robin).  Wave 0 never
+// waits for it.  Stops on wave 0's done flag or after kPrefetchRounds rounds.
+template <int EVAL>
+__device__ void spec_ahead(const DevIndex& ix, const uint64_t* stage, uint32_t ef_cap,
+                           const uint32_t* done, SpecLds* sp, const float* qv,
+                           const float* pmin, const float* pscale, uint32_t lane) {
+    const uint32_t deg = ix.deg;
+    if (deg > kWave) return;
+    uint32_t r = 0;
+    for (; r < kPrefetchRounds; ++r) {
+        if (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u) return;
+        if (__hip_atomic_load(&sp->qready, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u)
+            break;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    const uint32_t qq = uniform(sp->qq);
+    const uint32_t* qp = reinterpret_cast<const uint32_t*>(qv);
+    uint32_t next = 0;
+    for (; r < kPrefetchRounds; ++r) {
+        if (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u) break;
+        uint64_t key = ~0ull;
+        if (lane < ef_cap)
+            key = __hip_atomic_load(stage + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t id = key_id(key);
+        const uint32_t cur =
+            uniform(__hip_atomic_load(&sp->cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP));
+        bool open = (key & 1ull) == 0ull && id < ix.n && id != cur;
+#pragma unroll
+        for (uint32_t j = 0; j < kSpecSlots; ++j)
+            open = open && id != uniform(sp->slot[j].tag);
+        const uint64_t m = __ballot(open);
+        if (m == 0) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        const uint32_t x = uniform(__shfl(id, static_cast<int>(__builtin_ctzll(m))));
+        SpecSlot* sl = &sp->slot[next];
+        next = (next + 1u) % kSpecSlots;
+        if (lane == 0) __hip_atomic_store(&sl->tag, kInvalid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        uint32_t nid = kInvalid;
+        if (lane < deg) {
+            nid = ix.nbr[static_cast<size_t>(x) * deg + lane];
+            sl->ids[lane] = nid;
+        }
+        for (uint32_t cb = 0; cb < deg; cb += 32) {
+            // rows of ids past the index evaluate node x instead (never used)
+            if (lane >= cb && lane < cb + 32u) sp->bids[lane - cb] = nid < ix.n ? nid : x;
+            wave_sync();
+            uint64_t k;
+            uint32_t lg;
+            if constexpr (EVAL == EVAL_INT) {
+                k = eval_fresh_int<8>(ix, qp, qq, sp->bids, 32u, lane);
+                lg = 1u;
+            } else if constexpr (EVAL == EVAL_INT128) {
+                k = eval_fresh_int<4>(ix, qp, qq, sp->bids, 32u, lane);
+                lg = 1u;
+            } else {
+                k = eval_fresh(ix, qv, pmin, pscale, sp->bids, 32u, lane);
+                lg = fresh_lanes_log2(32u);
+            }
+            if ((lane & ((1u << lg) - 1u)) == 0) sl->keys[cb + (lane >> lg)] = k;
+            wave_sync();
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane == 0) __hip_atomic_store(&sl->tag, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+template <int EF_CAP, int EVAL>
+__global__ void __launch_bounds__(128) h1_spec_kernel(DevIndex ix, SearchJob job,
+                                                      uint32_t cache_bits, uint32_t dim_pad,
+                                                      uint32_t wave_bytes, uint32_t depth) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wid = uniform(threadIdx.x / kWave);
+    constexpr bool kParams = EVAL == EVAL_FLOAT;
+    float* pmin = nullptr;
+    float* pscale = nullptr;
+    if constexpr (kParams) {
+        pmin = reinterpret_cast<float*>(lds);
+        pscale = pmin + dim_pad;
+        for (uint32_t i = threadIdx.x; i < ix.dim; i += blockDim.x) {
+            pmin[i] = ix.sq8_min[i];
+            pscale[i] = ix.sq8_scale[i];
+        }
+    }
+    unsigned char* wb = lds + (kParams ? 8u * dim_pad : 0u);
+    WaveLds w;
+    w.qv = reinterpret_cast<float*>(wb);
+    w.stage = reinterpret_cast<uint64_t*>(wb + 4u * dim_pad);
+    w.bids = reinterpret_cast<uint32_t*>(w.stage + EF_CAP);
+    w.cache = w.bids + kWave;
+    uint32_t* done = reinterpret_cast<uint32_t*>(wb + wave_bytes);
+    SpecLds* sp = reinterpret_cast<SpecLds*>(wb + wave_bytes + 16u);
+    for (uint32_t i = threadIdx.x; i < static_cast<uint32_t>(EF_CAP); i += blockDim.x)
+        w.stage[i] = ~0ull;
+    if (threadIdx.x < kSpecSlots) sp->slot[threadIdx.x].tag = kInvalid;
+    if (threadIdx.x == 0) {
+        done[0] = 0u;
+        sp->qready = 0u;
+        sp->cur = kInvalid;
+    }
+    __syncthreads();
+    (void)depth;
+    const uint32_t q = blockIdx.x;
+    if (wid == 0) {
+        if (q < job.nq)
+            search_one<EF_CAP, EVAL, 1, true>(ix, job, q, pmin, pscale, w, cache_bits, lane,
+                                              nullptr, nullptr, sp);
+        if (lane == 0) __hip_atomic_store(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+        spec_ahead<EVAL>(ix, w.stage, static_cast<uint32_t>(EF_CAP), done, sp, w.qv, pmin, pscale,
+                         lane);
+    }
+}
+
 __global__ void reset_counter_kernel(uint32_t* counter, uint32_t n) {
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) counter[i] = 0u;
 }
@@ -965,7 +1150,25 @@ LatencyFn latency_for_eval(int ef_cap) {
     }
 }
 
+template <int EVAL>
+LatencyFn spec_for_eval(int ef_cap) {
+    switch (ef_cap) {
+        case 64: return &h1_spec_kernel<64, EVAL>;
+        case 128: return &h1_spec_kernel<128, EVAL>;
+        case 192: return &h1_spec_kernel<192, EVAL>;
+        case 256: return &h1_spec_kernel<256, EVAL>;
+        case 512: return &h1_spec_kernel<512, EVAL>;
+        case 1024: return &h1_spec_kernel<1024, EVAL>;
+        default: return nullptr;
+    }
+}
+
 LatencyFn select_latency(const LaunchCfg& cfg) {
+    if (cfg.prefetch == 4) {
+        if (cfg.eval == EVAL_INT128) return spec_for_eval<EVAL_INT128>(cfg.ef_cap);
+        return cfg.eval == EVAL_INT ? spec_for_eval<EVAL_INT>(cfg.ef_cap)
+                                    : spec_for_eval<EVAL_FLOAT>(cfg.ef_cap);
+    }
     if (cfg.eval == EVAL_INT128) return latency_for_eval<EVAL_INT128>(cfg.ef_cap);
     return cfg.eval == EVAL_INT ? latency_for_eval<EVAL_INT>(cfg.ef_cap)
                                 : latency_for_eval<EVAL_FLOAT>(cfg.ef_cap);
@@ -1059,12 +1262,12 @@ hipError_t launch_search(const DevIndex& ix, const SearchJob& job, const LaunchC
     if (cfg.prefetch > 0 && job.nq <= kLatencyMaxQueries && !job.visit_bits) {
         // latency mode: one query per workgroup (search wave + prefetch wave)
         LatencyFn k = select_latency(cfg);
-        if (!k || cfg.cache_bits < 4 || cfg.cache_bits > 14 || cfg.prefetch > 3)
+        if (!k || cfg.cache_bits < 4 || cfg.cache_bits > 14 || cfg.prefetch > 4)
             return hipErrorInvalidValue;
         uint32_t dim_pad, wave_bytes;
         geometry(ix, cfg, dim_pad, wave_bytes);
         const size_t lds = (cfg.eval == EVAL_FLOAT ? 8u * static_cast<size_t>(dim_pad) : 0u) +
-                           wave_bytes + 16u;
+                           wave_bytes + 16u + (cfg.prefetch == 4 ? sizeof(SpecLds) : 0u);
         if (lds > 64 * 1024) {
             hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                                hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -594,10 +594,12 @@ lqg_status lqg_set_launch(lqg_index* h, uint32_t ef, const lqg_launch* cfg) {
     if (!h) return fail(LQG_ERR_ARG, "null index");
     if (ef != 0 && ef_class(ef) == 0) return fail(LQG_ERR_UNSUPPORTED, "ef above 1024");
     if (cfg && (cfg->waves_per_group < 1 || cfg->waves_per_group > 4 || cfg->cache_bits < 4 ||
-                cfg->cache_bits > 14 || cfg->groups < 0 || cfg->prefetch < 0 || cfg->prefetch > 3 ||
+                cfg->cache_bits > 14 || cfg->groups < 0 || cfg->prefetch < 0 || cfg->prefetch > 4 ||
                 cfg->split < 0 || cfg->split == 3 || cfg->split > 4))
         return fail(LQG_ERR_ARG, "launch settings out of range");
     const bool fs = h->mode == LQG_MODE_FASTSCAN;
+    if (fs && cfg && cfg->prefetch > 3)
+        return fail(LQG_ERR_ARG, "FastScan prefetch depth must be in [0, 3]");
     if (fs && cfg && cfg->cache_bits < 6)
         return fail(LQG_ERR_ARG, "FastScan visited-table bits must be in [6, 14]");
     if (cfg && !fs && cfg->occupancy != -1 && cfg->occupancy != 0 && cfg->occupancy != 6 &&
@@ -801,7 +803,7 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
     // each prefetch depth (one query per workgroup: waves per group and the
     // register cap do not apply)
     if (small)
-        for (int pf : {1, 2, 3})
+        for (int pf : {1, 2, 3, 4})
             for (int cb : bits) {
                 lqg::LaunchCfg c = lqg::default_launch(cls, h->dim);
                 c.cache_bits = cb;
