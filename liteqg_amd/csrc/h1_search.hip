@@ -234,8 +234,10 @@ __device__ __forceinline__ uint64_t eval_fresh(const DevIndex& ix, const float* 
                                                const float* pmin, const float* pscale,
                                                const uint32_t* bids, uint32_t m,
                                                uint32_t lane) {
-    // blocks of NB chunks hold NB * max(1, PL / 4) row words per lane
-    constexpr int NB2 = NB > 16 ? NB / 2 : NB;
+    // blocks of NB chunks hold NB * max(1, PL / 4) row words per lane; two
+    // lanes per candidate (8 partials, two words per chunk) take half as many
+    // chunks, at most 16
+    constexpr int NB2 = NB > 8 ? (NB / 2 > 16 ? 16 : NB / 2) : NB;
     float r;
     const uint32_t lg = fresh_lanes_log2(m);
     const uint32_t L = 1u << lg;
@@ -327,10 +329,11 @@ struct SplitLds {
 
 // Chunks per block of row loads in the split kernel: 32 (one 32-bit word per
 // chunk and lane) for up to 4 partials per lane, 16 for 8 partials; half that
-// for pools of 512 or more entries, whose registers the pool takes.
-template <int EF_CAP>
+// for pools of 512 or more entries in the four-wave kernel, whose 128 VGPRs
+// the pool shares.
+template <int EF_CAP, int W>
 constexpr int split_nb() {
-    return EF_CAP >= 512 ? 16 : 32;
+    return W == 2 || EF_CAP < 512 ? 32 : 16;
 }
 
 // Wave wid's share of a pass of m fresh candidates: candidates [c0, c0 + mw)
@@ -396,7 +399,7 @@ constexpr int EVAL_INT128 = 3;
 // SPEC: wave 0 of a speculative latency-mode workgroup (h1_spec_kernel),
 // taking rows the other wave evaluated ahead from spec's slots.
 template <int EF_CAP, int EVAL, int W = 1, bool SPEC = false>
-__device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
+__device__ __forceinline__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
                            const float* pmin, const float* pscale, const WaveLds& w,
                            uint32_t cache_bits, uint32_t lane, uint32_t* vbits,
                            SplitLds* sp = nullptr, SpecLds* spec = nullptr) {
@@ -476,7 +479,7 @@ __device__ void search_one(const DevIndex& ix, const SearchJob& job, uint32_t q,
                 sp->go = 1u;
             }
             __syncthreads();  // the other waves start their shares
-            eval_share<W, split_nb<EF_CAP>()>(ix, w.qv, pmin, pscale, w.bids, m, 0u, lane,
+            eval_share<W, split_nb<EF_CAP, W>()>(ix, w.qv, pmin, pscale, w.bids, m, 0u, lane,
                                               sp->keys);
             __syncthreads();  // every share's keys are written
             return lane < m ? sp->keys[lane] : ~0ull;
@@ -725,50 +728,57 @@ __device__ __forceinline__ void search_kernel_body(const DevIndex& ix, const Sea
     if (gw >= sw) return;
     uint32_t* vbits =
         job.visit_bits ? job.visit_bits + static_cast<size_t>(gw) * job.visit_words : nullptr;
-    if (gridDim.x < kXcds || job.visit_bits) {
-        // one queue for the grid
-        uint32_t q = gw;
-        for (;;) {
-            q = uniform(q);
-            if (q >= job.nq) break;
-            search_one<EF_CAP, EVAL>(ix, job, q, pmin, pscale, w, cache_bits, lane, vbits);
-            uint32_t nx = 0;
-            if (lane == 0) nx = sw + atomicAdd(job.counter, 1u);
-            q = __shfl(nx, 0);
-        }
-        return;
-    }
-    // One queue per XCD.  Workgroup b runs on XCD b % 8 (round-robin
-    // dispatch); XCD x owns queries [x nq / 8, (x + 1) nq / 8): its waves take
-    // the first ones statically (wave j of the XCD takes q0 + j), the rest from
-    // the XCD's own counter (kQueueStride words apart, so the eight counters
-    // sit on separate cache lines and their atomics do not queue behind each
-    // other).  A wave whose range is drained moves on to the next XCD's
-    // counter, so the ranges drain evenly and every query is taken exactly
-    // once (each XCD has at least one workgroup, which takes its static part).
+    // One queue for the grid (small grids, counting launches): wave gw takes
+    // query gw, then sw + atomicAdd(counter).  Otherwise one queue per XCD.
+    // Workgroup b runs on XCD b % 8 (round-robin dispatch); XCD x owns
+    // queries [x nq / 8, (x + 1) nq / 8): its waves take the first ones
+    // statically (wave j of the XCD takes q0 + j), the rest from the XCD's
+    // own counter (kQueueStride words apart, so the eight counters sit on
+    // separate cache lines and their atomics do not queue behind each other).
+    // A wave whose range is drained moves on to the next XCD's counter, so the
+    // ranges drain evenly and every query is taken exactly once (each XCD has
+    // at least one workgroup, which takes its static part).  One call site of
+    // search_one for both.
+    const bool single = gridDim.x < kXcds || job.visit_bits;
     const uint32_t xcd = blockIdx.x % kXcds;
     const uint32_t lw = (blockIdx.x / kXcds) * wpg + wid;
-    for (uint32_t r = 0; r < kXcds; ++r) {
-        const uint32_t x = (xcd + r) % kXcds;
+    uint32_t r = 0;       // XCD queues visited
+    uint32_t base, q1;    // the current queue: next = base + atomicAdd(ctr), up to q1
+    uint32_t* ctr;
+    uint32_t q;
+    auto xcd_queue = [&](uint32_t x, uint32_t& q0) {
         const uint32_t nwx = ((gridDim.x + kXcds - 1u - x) / kXcds) * wpg;
-        const uint32_t q0 = static_cast<uint32_t>((static_cast<uint64_t>(x) * job.nq) / kXcds);
-        const uint32_t q1 =
-            static_cast<uint32_t>((static_cast<uint64_t>(x + 1u) * job.nq) / kXcds);
-        uint32_t* ctr = job.counter + kQueueStride * x;
-        uint32_t q;
-        if (r == 0) {
-            q = uniform(q0 + lw);
-        } else {
+        q0 = static_cast<uint32_t>((static_cast<uint64_t>(x) * job.nq) / kXcds);
+        q1 = static_cast<uint32_t>((static_cast<uint64_t>(x + 1u) * job.nq) / kXcds);
+        base = q0 + nwx;
+        ctr = job.counter + kQueueStride * x;
+    };
+    if (single) {
+        base = sw;
+        q1 = job.nq;
+        ctr = job.counter;
+        q = gw;
+    } else {
+        uint32_t q0;
+        xcd_queue(xcd, q0);
+        q = q0 + lw;
+    }
+    for (;;) {
+        q = uniform(q);
+        if (q >= q1) {
+            // this queue is drained: the next XCD's, until all are
+            if (single || ++r == kXcds) break;
+            uint32_t q0;
+            xcd_queue((xcd + r) % kXcds, q0);
             uint32_t nx = 0;
-            if (lane == 0) nx = q0 + nwx + atomicAdd(ctr, 1u);
-            q = uniform(__shfl(nx, 0));
+            if (lane == 0) nx = base + atomicAdd(ctr, 1u);
+            q = __shfl(nx, 0);
+            continue;
         }
-        while (q < q1) {
-            search_one<EF_CAP, EVAL>(ix, job, q, pmin, pscale, w, cache_bits, lane, vbits);
-            uint32_t nx = 0;
-            if (lane == 0) nx = q0 + nwx + atomicAdd(ctr, 1u);
-            q = uniform(__shfl(nx, 0));
-        }
+        search_one<EF_CAP, EVAL>(ix, job, q, pmin, pscale, w, cache_bits, lane, vbits);
+        uint32_t nx = 0;
+        if (lane == 0) nx = base + atomicAdd(ctr, 1u);
+        q = __shfl(nx, 0);
     }
 }
 
@@ -976,7 +986,7 @@ h1_split_kernel(DevIndex ix, SearchJob job,
         for (;;) {
             __syncthreads();
             if (uniform(sp->go) == 0u) break;
-            eval_share<W, split_nb<EF_CAP>()>(ix, w.qv, pmin, pscale, w.bids, uniform(sp->m),
+            eval_share<W, split_nb<EF_CAP, W>()>(ix, w.qv, pmin, pscale, w.bids, uniform(sp->m),
                                               wid, lane, sp->keys);
             __syncthreads();
         }

@@ -123,7 +123,11 @@ LaunchCfg default_launch(int ef_cap, uint64_t dim) {
     c.ef_cap = ef_cap;
     c.waves_per_group = 4;
     c.occupancy = 0;
-    c.cache_bits = 10;
+    // the visited cache must hold more of a query's touched nodes as ef
+    // grows: on 1,000 960-d queries at ef U{64..1024} (pool class 1024), 2^11
+    // entries evaluated 24.3 rows per expansion and 2^12 16.1 (18.8 -> 14.9
+    // ms per launch)
+    c.cache_bits = ef_cap >= 512 ? 12 : 10;
     c.groups = 0;
     // launches of up to kLatencyMaxQueries queries (a lone search()) run in
     // latency mode, loading the next expansions' id rows and SQ8 rows ahead:
@@ -766,10 +770,15 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
     const lqg::LaunchCfg saved = h->launch[slot];
     float best_ms = 1e30f;
     lqg::LaunchCfg best_cfg = lqg::default_launch(cls, h->dim);
-    // -1 (uncapped) differs from the default only up to the 256 class
+    // -1 (uncapped) differs from the default only up to the 256 class.  Pools
+    // of 512 entries and more take the uncapped kernel only: capped at 64 or
+    // 80 VGPRs they spill hundreds of registers (never the fastest), and one
+    // such variant with 16-chunk row blocks faulted in a tuning run
     const int occs[] = {0, 6, 8, -1};
-    const int n_occ = cls <= 256 ? 4 : 3;
-    const int bits[] = {8, 9, 10, 11};
+    const int n_occ = cls <= 256 ? 4 : 1;
+    // visited-cache sizes: 2^8 .. 2^11, and up to 2^13 for pools of 512 or more
+    const std::vector<int> bits =
+        cls >= 512 ? std::vector<int>{9, 10, 11, 12, 13} : std::vector<int>{8, 9, 10, 11};
     const int wpg[] = {1, 2, 4};
     // A tuning set larger than a latency-mode launch times the batch kernel
     // only, and keeps the class's prefetch depth for small launches (a lone
