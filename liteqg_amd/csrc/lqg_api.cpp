@@ -186,7 +186,14 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
     // with one wave per query; larger ones (a graph build's beam search of
     // every node: 300k x 960, 8.3 s unsplit against 8.6 s split) run the
     // batch kernel
-    if (!h->launch_set[class_slot(cls)] && nq > kSplitMaxQueries) cfg.split = 1;
+    if (!h->launch_set[class_slot(cls)] && nq > kSplitMaxQueries) {
+        cfg.split = 1;
+        // and the default visited cache of the high-ef classes is for launches
+        // of a few queries: a graph build's beam search of 1M nodes (EF 400,
+        // class 512) took 1.00 s per iteration with 2^12 entries, 0.32 s
+        // with 2^10 (LDS per wave sets its occupancy)
+        cfg.cache_bits = std::min(cfg.cache_bits, 10);
+    }
     // An index whose SQ8 decode is exact (min 0, scale 1; D % 16 == 0, D <=
     // 256) runs the integer-path kernel first; a float-path launch then takes
     // only the queries that were not integer valued.  Each kernel variant
