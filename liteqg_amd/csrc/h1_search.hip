@@ -90,6 +90,12 @@ __device__ __forceinline__ void chunk_block(const uint8_t* row, uint32_t t, cons
             const uint2 v = *reinterpret_cast<const uint2*>(src);
             wv[c][0] = v.x;
             wv[c][1] = v.y;
+        } else if constexpr (PL == 16) {
+            const uint4 v = *reinterpret_cast<const uint4*>(src);
+            wv[c][0] = v.x;
+            wv[c][1] = v.y;
+            wv[c][2] = v.z;
+            wv[c][3] = v.w;
         } else {
             wv[c][0] = *reinterpret_cast<const uint32_t*>(src);
         }
@@ -201,8 +207,16 @@ __device__ __forceinline__ float group_l2(const DevIndex& ix, const float* qv, c
 #pragma unroll
         for (int p = 0; p < 4; ++p) s[p] += __shfl_xor(s[p], 1);
         r = (s[0] + s[2]) + (s[1] + s[3]);
+    } else if constexpr (L == 1) {
+        // one lane holds all 16 partials: (p, p+8), (p, p+4), then
+        // (b0 + b2) + (b1 + b3)
+#pragma unroll
+        for (int p = 0; p < 8; ++p) s[p] += s[p + 8];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) s[p] += s[p + 4];
+        r = (s[0] + s[2]) + (s[1] + s[3]);
     } else {
-        static_assert(L == 2, "groups of 4 or 2 lanes");
+        static_assert(L == 2, "groups of 16, 8, 4, 2 or 1 lanes");
         // lane t holds partials 8t..8t+7: (p, p+8) across t^1, then in-lane
 #pragma unroll
         for (int p = 0; p < 8; ++p) s[p] += __shfl_xor(s[p], 1);
@@ -217,19 +231,21 @@ __device__ __forceinline__ float group_l2(const DevIndex& ix, const float* qv, c
 }
 
 // log2 of the lanes per candidate when a wave evaluates m fresh candidates:
-// 64 / (the smallest of 4, 8, 16, 32 that holds m).
+// 64 / (the smallest of 4, 8, 16, 32, 64 that holds m).
 __device__ __forceinline__ uint32_t fresh_lanes_log2(uint32_t m) {
-    return m <= 4 ? 4u : m <= 8 ? 3u : m <= 16 ? 2u : 1u;
+    return m <= 4 ? 4u : m <= 8 ? 3u : m <= 16 ? 2u : m <= 32 ? 1u : 0u;
 }
 
-// Keys of the m fresh candidates bids[0..m) (m <= 32) in one pass: candidate
-// c's key is returned on lane c*L, L = 1 << fresh_lanes_log2(m), so that few
-// candidates still spread over the whole wave (each lane then carries fewer
-// of the 16 partials, and the dependent chains are shorter); other lanes
-// return ~0.  All 64 lanes call it.  NB: chunks per block of row loads (8 in
-// the batch kernels; more in the split kernel, whose waves hold few
-// candidates each and have registers to spare).
-template <int NB = 8>
+// Keys of the m fresh candidates bids[0..m) (m <= 32, or <= 64 with WIDE) in
+// one pass: candidate c's key is returned on lane c*L, L = 1 <<
+// fresh_lanes_log2(m), so that few candidates still spread over the whole
+// wave (each lane then carries fewer of the 16 partials, and the dependent
+// chains are shorter); other lanes return ~0.  All 64 lanes call it.  NB:
+// chunks per block of row loads (8 in the batch kernels; more in the split
+// kernel, whose waves hold few candidates each and have registers to spare).
+// WIDE: one lane per candidate for 33..64 candidates (all 16 partials in the
+// lane, 16-byte loads).
+template <int NB = 8, bool WIDE = false>
 __device__ __forceinline__ uint64_t eval_fresh(const DevIndex& ix, const float* qv,
                                                const float* pmin, const float* pscale,
                                                const uint32_t* bids, uint32_t m,
@@ -250,8 +266,10 @@ __device__ __forceinline__ uint64_t eval_fresh(const DevIndex& ix, const float* 
         r = group_l2<8, NB>(ix, qv, pmin, pscale, id, act, lane & 7u);
     else if (L == 4)
         r = group_l2<4, NB>(ix, qv, pmin, pscale, id, act, lane & 3u);
-    else
+    else if (!WIDE || L == 2)
         r = group_l2<2, NB2>(ix, qv, pmin, pscale, id, act, lane & 1u);
+    else
+        r = group_l2<1, (NB > 8 ? 8 : NB)>(ix, qv, pmin, pscale, id, act, 0u);
     const bool leader = act && (lane & (L - 1)) == 0;
     return leader ? make_key(r, id) : ~0ull;
 }
@@ -308,6 +326,22 @@ __device__ __forceinline__ uint64_t eval_fresh_int(const DevIndex& ix, const uin
     return make_key(static_cast<float>(qq + cc - 2u * qc), id);
 }
 
+// eval_fresh_int for up to 64 candidates: candidates 32.. in a second call,
+// whose keys (on even lanes) move to the odd lanes.
+template <int NCH>
+__device__ __forceinline__ uint64_t eval_fresh_int_wide(const DevIndex& ix, const uint32_t* qp,
+                                                        uint32_t qq, const uint32_t* bids,
+                                                        uint32_t m, uint32_t lane) {
+    const uint64_t k1 = eval_fresh_int<NCH>(ix, qp, qq, bids, min(m, 32u), lane);
+    if (m <= 32u) return k1;
+    const uint64_t k2 = eval_fresh_int<NCH>(ix, qp, qq, bids + 32, m - 32u, lane);
+    const int src = static_cast<int>(lane) - 1;
+    const uint32_t lo = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(k2)), src));
+    const uint32_t hi =
+        static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(k2 >> 32)), src));
+    return (lane & 1u) ? ((static_cast<uint64_t>(hi) << 32) | lo) : k1;
+}
+
 struct WaveLds {
     float* qv;        // query (dim)
     uint64_t* stage;  // pool permutation staging (EF_CAP)
@@ -324,7 +358,7 @@ struct SplitLds {
     uint32_t go;
     uint32_t m;
     uint32_t pad[2];
-    uint64_t keys[kWave / 2];  // one per fresh candidate (at most 32 per pass)
+    uint64_t keys[kWave];  // one per fresh candidate (at most 64 per pass)
 };
 
 // Chunks per block of row loads in the split kernel: 32 (one 32-bit word per
@@ -468,8 +502,19 @@ __device__ __forceinline__ void search_one(const DevIndex& ix, const SearchJob& 
     uint64_t P[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) P[s] = ~0ull;
+    // Pools of 1024 entries (launches of a few high-ef queries) take the
+    // neighbours of a node of degree <= 64 in one pass of 64 ids: one id load
+    // and one evaluation instead of two.  Smaller pools keep passes of 32, and
+    // the registers that go with them (the 512 class, which graph builds at
+    // EF 500 run with a million queries, took 299 instead of 164 VGPRs).
+    constexpr bool kWide = EF_CAP >= 1024;
+    const uint32_t pass = kWide && ix.deg <= kWave ? kWave : 32u;
     auto evaluate = [&](uint32_t m) -> uint64_t {
-        if constexpr (EVAL == EVAL_INT)
+        if constexpr (EVAL == EVAL_INT && kWide)
+            return eval_fresh_int_wide<8>(ix, qp, qq, w.bids, m, lane);
+        else if constexpr (EVAL == EVAL_INT128 && kWide)
+            return eval_fresh_int_wide<4>(ix, qp, qq, w.bids, m, lane);
+        else if constexpr (EVAL == EVAL_INT)
             return eval_fresh_int<8>(ix, qp, qq, w.bids, m, lane);
         else if constexpr (EVAL == EVAL_INT128)
             return eval_fresh_int<4>(ix, qp, qq, w.bids, m, lane);
@@ -484,7 +529,7 @@ __device__ __forceinline__ void search_one(const DevIndex& ix, const SearchJob& 
             __syncthreads();  // every share's keys are written
             return lane < m ? sp->keys[lane] : ~0ull;
         } else {
-            return eval_fresh(ix, w.qv, pmin, pscale, w.bids, m, lane);
+            return eval_fresh<8, kWide>(ix, w.qv, pmin, pscale, w.bids, m, lane);
         }
     };
     {
@@ -522,6 +567,7 @@ __device__ __forceinline__ void search_one(const DevIndex& ix, const SearchJob& 
         bool use_spec = false;
         uint32_t s_id0 = kInvalid, s_id1 = kInvalid;
         uint64_t s_k0 = ~0ull, s_k1 = ~0ull;
+        // (with passes of 64, lane l holds neighbour l in s_id0, s_k0)
         if constexpr (SPEC) {
             const uint32_t x = key_id(kc);
             if (lane == 0) __hip_atomic_store(&spec->cur, x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -532,13 +578,13 @@ __device__ __forceinline__ void search_one(const DevIndex& ix, const SearchJob& 
             const uint64_t hit = __ballot(lane < kSpecSlots && t == x);
             if (hit && ix.deg <= kWave) {
                 SpecSlot* sl = &spec->slot[__builtin_ctzll(hit)];
-                if (lane < 32) {
+                if (lane < pass && lane < ix.deg) {
                     s_id0 = sl->ids[lane];
                     s_k0 = sl->keys[lane];
-                    if (32u + lane < ix.deg) {
-                        s_id1 = sl->ids[32u + lane];
-                        s_k1 = sl->keys[32u + lane];
-                    }
+                }
+                if (pass == 32u && lane < 32 && 32u + lane < ix.deg) {
+                    s_id1 = sl->ids[32u + lane];
+                    s_k1 = sl->keys[32u + lane];
                 }
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
                 const uint32_t t2 =
@@ -547,11 +593,11 @@ __device__ __forceinline__ void search_one(const DevIndex& ix, const SearchJob& 
             }
         }
 
-        for (uint32_t cb = 0; cb < ix.deg; cb += 32) {
-            // neighbours not seen before (qg.hpp:320-324), 32 ids at a time
+        for (uint32_t cb = 0; cb < ix.deg; cb += pass) {
+            // neighbours not seen before (qg.hpp:320-324), `pass` ids at a time
             bool fresh = false;
             uint32_t nid = kInvalid;
-            if (lane < 32 && cb + lane < ix.deg) {
+            if (lane < pass && cb + lane < ix.deg) {
                 nid = use_spec ? (cb == 0 ? s_id0 : s_id1) : nrow[cb + lane];
                 if (nid < ix.n) {
                     const uint32_t slot = (nid * 2654435761u) >> (32 - cache_bits);

@@ -195,6 +195,42 @@ def test_split_evaluation_matches_oracle(oracle_mod, tmp_path, split, kind, d, d
     assert np.array_equal(np.stack([ix.search(q, 10) for q in Q[:10]]), exp)
 
 
+@pytest.mark.parametrize("kind,d,dup", [("int", 128, 0.0), ("uniform", 96, 0.1),
+                                         ("uniform", 100, 0.0), ("uniform", 960, 0.0)])
+def test_wide_pass_matches_oracle(oracle_mod, tmp_path, kind, d, dup):
+    """The 1024-entry pool class takes a degree-64 node's neighbours in one
+    pass of 64 ids (up to 64 fresh candidates evaluated together, one lane
+    each past 32; the integer path in two calls): ids and distance bits equal
+    the oracle's in both orders at ef 513..1024, k 10 and 100, with duplicate
+    ids in the rows, in the batch kernel, the split kernel, latency mode
+    (prefetch 2) and speculative latency mode (prefetch 4)."""
+    n = 2500 if d <= 128 else 1200
+    arr, base = make_random_index(n, d, 64, seed=d + 11, kind=kind, knn=True, dup_rate=dup)
+    ix = device_from_arrays(arr, tmp_path)
+    rng = np.random.default_rng(d + 12)
+    nq = 40
+    Q = (base[rng.integers(0, n, nq)] + rng.normal(0, 0.05 if kind == "uniform" else 10,
+                                                   size=(nq, d))).astype(np.float32)
+    if kind == "int":
+        Q = np.clip(np.rint(Q), 0, 255).astype(np.float32)
+    settings = [dict(waves_per_group=1, cache_bits=10), dict(waves_per_group=4, cache_bits=12),
+                dict(waves_per_group=1, cache_bits=10, prefetch=2),
+                dict(waves_per_group=1, cache_bits=10, prefetch=4)]
+    if kind != "int":
+        settings.append(dict(waves_per_group=1, cache_bits=11, split=2))
+    for cfg in settings:
+        ix.set_launch(1024, **cfg)
+        for ef, k, order in ((600, 10, "nearest"), (1024, 100, "reference"), (777, 10, "reference")):
+            for m in (1, nq):
+                ids, dis = ix.search_batch(Q[:m], k, ef=ef, order=order, return_distances=True)
+                exp, exp_d, _ = oracle_batch(oracle_mod, arr, Q[:m], ef, k, order)
+                assert_same(ids, dis, exp, exp_d)
+    ix.set_ef(1024)
+    ix.set_order("nearest")
+    exp, _, _ = oracle_batch(oracle_mod, arr, Q[:5], 1024, 10, "nearest")
+    assert np.array_equal(np.stack([ix.search(q, 10) for q in Q[:5]]), exp)
+
+
 def test_split_is_default_at_high_dimension(oracle_mod, tmp_path):
     """An index of D >= 512 splits its float-path evaluation over two waves
     by default; autotuning on it considers split launches and keeps results."""
