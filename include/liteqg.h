@@ -267,7 +267,9 @@ typedef struct {
                                 64 VGPRs), for indexes with P / lanes per
                                 neighbour <= 128 code bits (8: k <= 64)          */
     int32_t groups;          /* workgroups of the persistent grid; 0 = fill the device  */
-    int32_t prefetch;        /* launches of at most 256 queries (a lone search()):
+    int32_t prefetch;        /* launches of at most 256 queries (a lone search()),
+                                and SQ8 launches of at most 1,024 on indexes of
+                                at most 256 dims:
                                 0 = the batch kernel; 1, 2 or 3 = latency mode, one
                                 query per workgroup with a second wave that loads
                                 ahead what the next expansions read.  SQ8 -- 1: the

@@ -1315,7 +1315,7 @@ hipError_t launch_search(const DevIndex& ix, const SearchJob& job, const LaunchC
                            static_cast<uint32_t>(cfg.cache_bits), dim_pad, wave_bytes);
         return hipGetLastError();
     }
-    if (cfg.prefetch > 0 && job.nq <= kLatencyMaxQueries && !job.visit_bits) {
+    if (cfg.prefetch > 0 && job.nq <= sq8_latency_max(ix.dim) && !job.visit_bits) {
         // latency mode: one query per workgroup (search wave + prefetch wave)
         LatencyFn k = select_latency(cfg);
         if (!k || cfg.cache_bits < 4 || cfg.cache_bits > 14 || cfg.prefetch > 4)

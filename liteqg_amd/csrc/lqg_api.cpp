@@ -129,8 +129,8 @@ LaunchCfg default_launch(int ef_cap, uint64_t dim) {
     // ms per launch)
     c.cache_bits = ef_cap >= 512 ? 12 : 10;
     c.groups = 0;
-    // launches of up to kLatencyMaxQueries queries (a lone search()) run in
-    // latency mode, loading the next expansions' id rows and SQ8 rows ahead:
+    // launches of up to sq8_latency_max(dim) queries (a lone search(); 1,024
+    // queries on rows of at most 256 dims, else 256) run in latency mode, loading the next expansions' id rows and SQ8 rows ahead:
     // on the 1M x 128 workload at ef 50 a lone search() went from p50 0.115-
     // 0.120 ms to 0.100 ms, a 256-query batch from 0.216 to 0.186 ms
     c.prefetch = 2;
@@ -219,7 +219,7 @@ lqg_status run_batch(lqg_index* h, const float* d_queries, uint64_t nq, const ui
     // a single latency-mode or split launch (one query per workgroup) uses no queue
     const bool one_launch = !count_visits && (!int_index || queries_int == 0 || queries_int == 1);
     const bool float_only = !int_index || queries_int == 0;
-    if (!(one_launch && ((cfg.prefetch > 0 && nq <= kLatencyMaxQueries) ||
+    if (!(one_launch && ((cfg.prefetch > 0 && nq <= sq8_latency_max(h->dim)) ||
                          (float_only && cfg.split > 1))))
         LQG_HIP_TRY(launch_reset_counter(sc->counter.p, s, static_cast<uint32_t>(nwords)));
     uint32_t* left = sc->counter.p + 2 * kQueueWords;
@@ -790,7 +790,7 @@ lqg_status lqg_autotune(lqg_index* h, const float* queries, uint64_t nq, uint32_
     // A tuning set larger than a latency-mode launch times the batch kernel
     // only, and keeps the class's prefetch depth for small launches (a lone
     // search() after tuning on a large batch still runs in latency mode).
-    const bool small = nq <= lqg::kLatencyMaxQueries;
+    const bool small = nq <= lqg::sq8_latency_max(h->dim);
     const int keep_pf = had ? saved.prefetch : lqg::default_launch(cls, h->dim).prefetch;
     std::vector<lqg::LaunchCfg> cands;
     for (int oi = 0; oi < n_occ; ++oi)

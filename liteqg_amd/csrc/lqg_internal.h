@@ -83,16 +83,29 @@ struct LaunchCfg {
     int occupancy;       // register cap: 8 = 64 VGPRs, 6 = 80 VGPRs, 0 = uncapped
     int eval = 0;        // 0: SQ8 float distances; 1: exact integer distances (v_dot4);
                          // 3: the same for rows of at most 128 bytes
-    int prefetch = 0;    // launches of at most kLatencyMaxQueries queries: 0 = the batch
-                         // kernel; 1..3 = latency mode with that prefetch depth
+    int prefetch = 0;    // launches of at most sq8_latency_max(dim) queries: 0 = the
+                         // batch kernel; 1..4 = latency mode with that prefetch depth
     int split = 1;       // SQ8 float path: waves per query sharing each pass's
                          // distance evaluation (h1_split_kernel); 1 = off, 2 or 4
 };
 
 // Largest launch that takes the latency-mode kernel (one query per
 // workgroup, a prefetch wave beside each search wave) when its class's
-// prefetch depth is set.
+// prefetch depth is set: FastScan, and the SQ8 path on rows of more than 256
+// dims.  On 1,000-query launches latency mode was slower for 960-d rows in
+// both modes (FastScan 4.56 -> 4.74 ms; SQ8 without split evaluation 11.3 ->
+// 19.6 ms, 200k x 960 degree 64, ef U{64..1024}).
 constexpr uint32_t kLatencyMaxQueries = 256;
+
+// The same limit for the SQ8 path on rows of at most 256 dims: 1,000
+// queries of 100k x 64 uniform, latency mode depth 2 against the batch
+// kernel at ef 50 / 100 / 200: 0.191 / 0.319 / 0.614 -> 0.175 / 0.291 /
+// 0.598 ms.
+constexpr uint32_t kLatencyMaxQueriesNarrow = 1024;
+
+__host__ __device__ inline uint32_t sq8_latency_max(uint32_t dim) {
+    return dim <= 256u ? kLatencyMaxQueriesNarrow : kLatencyMaxQueries;
+}
 
 // Device view of an index for the FastScan RaBitQ search (search_qg,
 // qg.hpp:252-287).  One record per node, `stride` bytes (a multiple of 128):

@@ -154,7 +154,8 @@ class Index:
                    occupancy: int = 0, groups: int = 0, prefetch: int = 0,
                    split: int = 1) -> None:
         """Kernel launch settings for ef's capacity class (ef=0: every class).
-        prefetch (SQ8 launches of at most 256 queries, e.g. a lone search()):
+        prefetch (SQ8 launches of at most 256 queries, e.g. a lone search(), or
+        1,024 on an index of at most 256 dims):
         0 = the batch kernel, 1..3 = latency mode with that prefetch depth.
         split (SQ8 float path): 1 = off, 2 or 4 waves per query sharing each
         expansion's distance evaluation (takes precedence over prefetch)."""

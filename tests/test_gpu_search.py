@@ -127,20 +127,22 @@ def test_random_index_parity(oracle_mod, tmp_path, case):
                                            ("uniform", 768, 32, 100)])
 def test_latency_mode_matches_oracle(oracle_mod, tmp_path, prefetch, kind, d, deg, ef):
     """Latency mode (one query per workgroup, a second wave loading ahead, or
-    at depth 4 evaluating ahead): batches of up to 256 queries and the plain
-    per-query search give the oracle's ids and distance bits in both orders;
-    a larger batch runs the batch kernel with the same results."""
+    at depth 4 evaluating ahead): batches of up to 256 queries (1,024 at
+    D <= 256) and the plain per-query search give the oracle's ids and
+    distance bits in both orders; a larger batch runs the batch kernel with
+    the same results."""
     arr, base = make_random_index(3000, d, deg, seed=d + prefetch, kind=kind, knn=True)
     ix = device_from_arrays(arr, tmp_path)
     ix.set_launch(ef, waves_per_group=1, cache_bits=10, prefetch=prefetch)
     assert ix.get_launch(ef)["prefetch"] == prefetch
     rng = np.random.default_rng(5)
-    Q = (base[rng.integers(0, 3000, 300)] + rng.normal(0, 0.05 if kind == "uniform" else 10,
-                                                       size=(300, d))).astype(np.float32)
+    nmax = 1025 if d <= 256 else 300
+    Q = (base[rng.integers(0, 3000, nmax)] + rng.normal(0, 0.05 if kind == "uniform" else 10,
+                                                        size=(nmax, d))).astype(np.float32)
     if kind == "int":
         Q = np.clip(np.rint(Q), 0, 255).astype(np.float32)
     for order in ("reference", "nearest"):
-        for nq in (1, 37, 256, 300):
+        for nq in (1, 37, 256, 300) + ((1024, 1025) if d <= 256 else ()):
             ids, dis = ix.search_batch(Q[:nq], 10, ef=ef, order=order, return_distances=True)
             exp, exp_d, _ = oracle_batch(oracle_mod, arr, Q[:nq], ef, 10, order)
             assert_same(ids, dis, exp, exp_d)
