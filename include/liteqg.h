@@ -166,8 +166,8 @@ lqg_status lqg_search_batch_device(lqg_index* index, const float* d_queries, uin
  * buffers the library allocates.  Limits:
  *   - the index must already be on the device (any earlier search on it);
  *   - FastScan mode has no device overflow visited tables: a query whose
- *     visited set passes half of its LDS table (256 to 2,048 expansions by pool
- *     class) fails as in lqg_h2_set_overflow_bits(index, 0);
+ *     visited set passes three quarters of its LDS table (384 to 3,072
+ *     expansions by pool class) fails as in lqg_h2_set_overflow_bits(index, 0);
  *   - with LQG_ROTATION_GEMM, nq x P must be at most 262,144 (LQG_ERR_ARG).
  * Calls share the GPU's array: two calls that overlap on one GPU race. */
 lqg_status lqg_search_batch_device_noalloc(lqg_index* index, const float* d_queries,
@@ -385,10 +385,11 @@ lqg_status lqg_h2_expansions(lqg_index* index, const float* queries, uint64_t nq
                              uint32_t cap, uint32_t* out);
 
 /* FastScan visited set: each query's exact set lives in an LDS table sized by
- * its pool class and moves, past half full, to a per-wave table of 2^bits
- * entries in device memory (default 14); past half of that the query fails
+ * its pool class and moves, past three quarters full, to a per-wave table of
+ * 2^bits entries in device memory (default 14); past three quarters of that
+ * the query fails
  * (E = 0xFFFFFFFF in d_stats, ids 0xFFFFFFFF, its own serving request fails).
- * bits = 0: no device table, a query fails once its LDS table is half full
+ * bits = 0: no device table, a query fails once its LDS table is 3/4 full
  * (for tests of that failure path); otherwise 8 <= bits <= 16.  No
  * counterpart in the reference, whose hash set grows without bound
  * (third/ngt/hashset.hpp:30-110). */

@@ -103,14 +103,13 @@ __device__ __forceinline__ uint32_t scan_sum_grouped(const uint32_t (&cw)[CW], L
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < CW; ++k) {
-        const uint32_t t = cw[k] & 0x0F0F0F0Fu, v = cw[k] & 0xF0F0F0F0u;
+        const uint32_t t = cw[k] & 0x0F0F0F0Fu, v = (cw[k] >> 4) & 0x0F0F0F0Fu;
         uint32_t e[4], o[4];
 #define LQG_SCAN_BYTE2(J)                                                                   \
     {                                                                                       \
         const uint32_t b = 4u * static_cast<uint32_t>(k) + (J);                             \
         e[J] = reinterpret_cast<lds_u8*>(static_cast<uintptr_t>(add_byte<J>(lb.even, t)))[16u * b]; \
-        o[J] = reinterpret_cast<lds_u8*>(static_cast<uintptr_t>(add_byte<J>(lb.odd, v)))   \
-            [256u * (b >> 4) + (b & 15u)];                                                  \
+        o[J] = reinterpret_cast<lds_u8*>(static_cast<uintptr_t>(add_byte<J>(lb.odd, v)))[16u * b]; \
     }
         LQG_SCAN_BYTE2(0)
         LQG_SCAN_BYTE2(1)
@@ -182,7 +181,7 @@ __device__ __forceinline__ uint32_t v_test_set_fast(Visited<false>& v, uint32_t 
     // a hit before the first empty slot (all of them when there is none)
     const uint64_t first_emp = emp & (0ull - emp);
     if (hit & (first_emp - 1ull)) return kVisSeen;
-    if (v.count + 1u > (1u << v.bits) / 2u) return kVisFull;
+    if (v.count + 1u > vis_limit(v.bits)) return kVisFull;
     if (lane == static_cast<uint32_t>(__builtin_ctzll(emp))) v.tab[slot] = id;
     wave_sync();
     ++v.count;
@@ -296,7 +295,7 @@ __device__ bool h2_fast_one(const H2Dev& ix, const H2Job& job, uint32_t q, const
     U[0] = 1;
     uint64_t last = ~0ull;  // key of entry ef - 1 once the pool is full
     uint32_t size = 1, seq = 1, n_exp = 0, n_pop = 0;
-    bool full = false;  // the visited set would pass half of its table
+    bool full = false;  // the visited set would pass vis_limit of its table
     ResBuf<KS> rb;
     rb_init(rb);
     float qr[NJ];
@@ -447,7 +446,7 @@ __device__ bool h2_fast_one(const H2Dev& ix, const H2Job& job, uint32_t q, const
         }
     }
 
-    // a query whose visited set would pass half of the LDS table goes to the
+    // a query whose visited set would pass vis_limit of the LDS table goes to the
     // overflow list, or fails where the launch has none
     if (full && job.ovf) return h2_defer(job, q, lane);
     const bool failed = full;

@@ -29,7 +29,7 @@
 // (seq = insertion number) and merges each node's neighbours in one pass, as
 // the SQ8 kernel does.  The visited set (nodes expanded, plus the nodes
 // update_results adds) is exact: an LDS open-addressing table, moved to a
-// per-wave table in device memory if it passes half its capacity.
+// per-wave table in device memory if it passes three quarters of its capacity.
 //
 // FastScan: a lane holds the codes of one neighbour (deg >= 64) or half of
 // them (deg = 32) and adds LUT bytes looked up in LDS; sums are taken mod
@@ -52,6 +52,9 @@ struct Layout {
 
 __host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
 
+// Bytes of a query's LUT in LDS (layout: lut_pos below).
+__host__ __device__ inline uint32_t lut_bytes(uint32_t p) { return 4u * p + 32u; }
+
 // Per-wave LDS layout.  with_qv: keep the query in LDS (the probe kernel, and
 // searches with dim > 16 * kRegJ, whose exact distances read dims past the
 // registers' from there).
@@ -61,7 +64,7 @@ __host__ __device__ inline Layout h2_layout(uint32_t dim, uint32_t p, uint32_t e
     uint32_t o = 0;
     l.has_qv = with_qv ? 1 : 0;
     l.qv = o;   o += with_qv ? align16(4 * dim) : 0u;
-    l.lut = o;  o += align16(2u * p + 256u * ((p / 8u + 15u) / 16u));
+    l.lut = o;  o += align16(lut_bytes(p));
     l.pl = o;   o += align16(p);  // bit planes: 8 words per 32-bit code word
     // one region for the preparation buffers (tb, qb) and, once the LUT is
     // built, the pool merge staging (ef_cap records and a spare)
@@ -100,7 +103,7 @@ struct H2Lds {
     float* qv;        // query (dim), only where dim > 16 * kRegJ (the rest sits in registers)
     float* tb;        // rotated query during preparation; q - x in exact_l2 (p floats)
     uint8_t* qb;      // quantised query (p bytes)
-    uint8_t* lut;     // 4p bytes (+16 bytes of bank skew for 2 lanes per vector)
+    uint8_t* lut;     // lut_bytes(p): 4p bytes and 2 x 16 bytes of bank skew
     uint4* pl;        // query bit planes (see prepare): 2 uint4 per 32-bit code word
     uint4* st;        // pool merge staging, once the LUT is built (shares tb's memory):
                       // EF_CAP + 1 records {key lo, key hi, id, 0}
@@ -122,17 +125,18 @@ __device__ __forceinline__ H2Lds carve(unsigned char* base, const Layout& l) {
 // ---------------------------------------------------------------------------
 // LUT layout for the scan.  A code byte holds codebook 2g (low nibble) and
 // 2g + 1 (high nibble) of codebook pair g.  Even codebooks: 16 bytes per pair,
-// row after row (pair g's entry j at 16g + j).  Odd codebooks: column g % 16
-// of a 16 x 16 block per 16 pairs (entry j at 2p + 256 (g / 16) + 16 j +
-// g % 16), so that the high nibble, masked in place (16 j), is the offset.
-// 4p bytes (2p + 256 for p = 64).
-__host__ __device__ inline uint32_t lut_pos(uint32_t p, uint32_t c, uint32_t j) {
+// row after row (pair g's entry j at 16g + j); odd codebooks: the same rows
+// from 2p + 16 on.  With two lanes per vector (lpv = 2) each lane scans half
+// of the pairs, and the rows of the second half sit 16 bytes further on.  The
+// LDS serves a wavefront in two 32-lane passes, and two dwords 32 dwords
+// apart share a bank: without the skew, the rows the two lanes of a vector
+// read at once (pairs g and g + p/16, 2p / 16 bytes apart) share banks, which
+// an LDS probe of exactly these lookups measured at 1.9 conflict cycles per
+// lookup, against none with the skew.  4p + 32 bytes.
+__host__ __device__ inline uint32_t lut_pos(uint32_t p, uint32_t c, uint32_t j, uint32_t lpv) {
     const uint32_t g = c >> 1;
-    return (c & 1u) ? 2u * p + 256u * (g >> 4) + 16u * j + (g & 15u) : 16u * g + j;
-}
-
-__host__ __device__ inline uint32_t lut_bytes(uint32_t p) {
-    return 2u * p + 256u * ((p / 8u + 15u) / 16u);
+    const uint32_t skew = (lpv == 2u && g >= p / 16u) ? 16u : 0u;
+    return (c & 1u) * (2u * p + 16u) + 16u * g + skew + j;
 }
 
 // Query preparation (qg_query.hpp:28-43) into LDS: qv (raw query), lut.
@@ -204,16 +208,15 @@ __device__ Prep prepare(const H2Dev& ix, const float* qsrc, const float* rsrc, c
     wave_sync();
     // pack_lut_impl (fastscan_impl.hpp:258-270): entry j of codebook c is the
     // uint8 sum of q[4c + 3 - b] over the set bits b of j.  Stored at
-    // lut_pos(p, c, j) (the scan's layout, lut_pos).
+    // lut_pos(p, c, j, lpv) (the scan's layout).
     const uint32_t ncb = p / 4;
-    (void)lpv;
     for (uint32_t idx = lane; idx < 16u * ncb; idx += kWave) {
         const uint32_t c = idx >> 4, j = idx & 15u;
         uint32_t s = 0;
 #pragma unroll
         for (uint32_t b = 0; b < 4; ++b)
             if ((j >> b) & 1u) s += w.qb[4 * c + 3 - b];
-        w.lut[lut_pos(p, c, j)] = static_cast<uint8_t>(s & 0xFFu);
+        w.lut[lut_pos(p, c, j, lpv)] = static_cast<uint8_t>(s & 0xFFu);
     }
     // Bit planes.  Code bit i of 32-bit code word W (little-endian bytes
     // 4W .. 4W + 3) stands for dim d(W, i) = 8B + r, B = 4W + i / 8, with
@@ -350,7 +353,7 @@ __device__ __forceinline__ float exact_l2_regs(const float (&xr)[kRegJ], const f
 // ---------------------------------------------------------------------------
 // Exact visited set (nodes expanded, and the nodes update_results adds):
 // open addressing over 2^bits slots (empty = 0xFFFFFFFF), in LDS or, for a
-// query whose set would pass half of its LDS table, in a per-wave table in
+// query whose set would pass vis_limit of its LDS table, in a per-wave table in
 // device memory (the query is then searched again from the start with that
 // table: GLOBAL = true).  An id's probe sequence starts at the first slot of
 // its 4-slot bucket and runs through consecutive slots.  Nothing is ever
@@ -375,9 +378,15 @@ __device__ __forceinline__ uint32_t g_load(const uint32_t* p) {
 
 enum : uint32_t { kVisNew = 0, kVisSeen = 1, kVisFull = 2 };
 
+// Most ids a table of 2^bits slots takes: three quarters of it, so that every
+// probe meets an empty slot within a few buckets.  (At half, the 2^11-slot
+// table of the 1,024-entry pool class turned away the queries at ef near
+// 1,024, which expand about 1,030 nodes, and they were searched twice.)
+__host__ __device__ inline uint32_t vis_limit(uint32_t bits) { return 3u << (bits - 2u); }
+
 // For an id equal on every lane: kVisSeen if it is in the set; otherwise
-// inserts it (kVisNew), or, if the set would pass half of its table, inserts
-// nothing (kVisFull).
+// inserts it (kVisNew), or, if the set would pass vis_limit, inserts nothing
+// (kVisFull).
 template <bool GLOBAL>
 __device__ uint32_t v_test_set(Visited<GLOBAL>& v, uint32_t id, uint32_t lane) {
     const uint32_t mask = (1u << v.bits) - 1u;
@@ -395,7 +404,7 @@ __device__ uint32_t v_test_set(Visited<GLOBAL>& v, uint32_t id, uint32_t lane) {
         }
         h = (h + kWave) & mask;
     }
-    if (v.count + 1u > (1u << v.bits) / 2u) return kVisFull;
+    if (v.count + 1u > vis_limit(v.bits)) return kVisFull;
     if (GLOBAL) {
         // the first empty slot may have been taken meanwhile by no one else
         // (the table is this wave's), so the CAS lands there
@@ -539,7 +548,8 @@ __device__ __forceinline__ uint32_t add_byte(uint32_t base, uint32_t t) {
 typedef __attribute__((address_space(3))) const uint8_t lds_u8;
 
 // The two LDS base addresses of a lane's LUT lookups (lut_pos for the lane's
-// first codebook pair, 4 CW pairs per lane, `part` of lpv lanes per vector).
+// first codebook pair, 4 CW pairs per lane, `part` of lpv lanes per vector;
+// part 1 only where lpv = 2, and then its pairs are the skewed half).
 struct LutBase {
     uint32_t even, odd;
 };
@@ -548,25 +558,24 @@ template <int CW>
 __device__ __forceinline__ LutBase lut_base(const uint8_t* lut, uint32_t p, uint32_t part) {
     // the LDS offset of the generic pointer (an address-space cast)
     const uint32_t l = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((lds_u8*)lut));
-    const uint32_t g0 = 4u * CW * part;
-    return LutBase{l + 16u * g0, l + 2u * p + 256u * (g0 >> 4) + (g0 & 15u)};
+    const uint32_t first = 16u * (4u * CW * part) + (part ? 16u : 0u);
+    return LutBase{l + first, l + 2u * p + 16u + first};
 }
 
 // FastScan sum of one lane: CW code words, pairs g0 .. g0 + 4 CW - 1 of the
-// lane's LUT bases: per word, the low nibbles masked in place index the even
-// rows, the high nibbles masked in place (16 j) the odd columns.
+// lane's LUT bases: per word, the low nibbles index the even rows and the
+// high nibbles, shifted down, the odd rows.
 template <int CW>
 __device__ __forceinline__ uint32_t scan_sum(const uint32_t (&cw)[CW], LutBase lb) {
     uint32_t s = 0;
 #pragma unroll
     for (int k = 0; k < CW; ++k) {
-        const uint32_t t = cw[k] & 0x0F0F0F0Fu, v = cw[k] & 0xF0F0F0F0u;
+        const uint32_t t = cw[k] & 0x0F0F0F0Fu, v = (cw[k] >> 4) & 0x0F0F0F0Fu;
 #define LQG_SCAN_BYTE(J)                                                                     \
     {                                                                                       \
         const uint32_t b = 4u * static_cast<uint32_t>(k) + (J);                             \
         s += reinterpret_cast<lds_u8*>(static_cast<uintptr_t>(add_byte<J>(lb.even, t)))[16u * b]; \
-        s += reinterpret_cast<lds_u8*>(static_cast<uintptr_t>(add_byte<J>(lb.odd, v)))     \
-            [256u * (b >> 4) + (b & 15u)];                                                  \
+        s += reinterpret_cast<lds_u8*>(static_cast<uintptr_t>(add_byte<J>(lb.odd, v)))[16u * b]; \
     }
         LQG_SCAN_BYTE(0)
         LQG_SCAN_BYTE(1)
@@ -644,12 +653,12 @@ __device__ __forceinline__ float lane_estimate(const uint32_t (&cw)[CW], LutBase
 
 // ---------------------------------------------------------------------------
 // One query.  GLOBAL = false (the search kernel): the visited set in the
-// wave's LDS table; if it would pass half of it, the query is appended to
+// wave's LDS table; if it would pass vis_limit, the query is appended to
 // the launch's overflow list and false returned with nothing written (the
 // overflow kernel searches it again from the start), or, where the launch
 // has no overflow list, the query fails.  GLOBAL = true (the overflow
 // kernel): the set in the wave's device table gtab.  A failed query (past
-// half of its last table, or no device table) has E = 0xFFFFFFFF in the
+// vis_limit of its last table, or no device table) has E = 0xFFFFFFFF in the
 // stats and ids 0xFFFFFFFF, sets job.err and its kFlagFailed flag.
 // Appends query q to the launch's overflow list (ovf[0] = count, ovf[2 + i]
 // = entries); returns false.

@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(64) h2_probe_kernel(H2Dev ix, const float* que
     const Prep pr = prepare(ix, queries + static_cast<size_t>(q) * ix.dim, nullptr, w, lpv, lane);
     // LUT in the reference's layout (entry j of codebook c at 16 c + j)
     for (uint32_t idx = lane; idx < 4 * ix.p; idx += kWave)
-        lut_out[static_cast<size_t>(q) * 4 * ix.p + idx] = w.lut[lut_pos(ix.p, idx >> 4, idx & 15u)];
+        lut_out[static_cast<size_t>(q) * 4 * ix.p + idx] = w.lut[lut_pos(ix.p, idx >> 4, idx & 15u, lpv)];
     if (lane == 0) {
         params[3 * q] = pr.lo;
         params[3 * q + 1] = pr.width;

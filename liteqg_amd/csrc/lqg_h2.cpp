@@ -164,7 +164,8 @@ H2Cfg h2_default_launch(int ef_cap) {
     c.waves_per_group = 4;
     c.groups = 0;
     // LDS visited table: at least 4x the largest ef of the class, so that
-    // expansions (typically 1x-3x ef) stay under half of it
+    // expansions (typically 1x-3x ef) stay under half of it (the limit is
+    // three quarters, vis_limit)
     c.hash_bits = ef_cap <= 64 ? 9 : ef_cap <= 256 ? 10 : ef_cap <= 512 ? 11 : 12;
     // 6 waves per SIMD up to the 192 class: with these defaults on the 1M x 128
     // workload, 4-12% faster than uncapped at ef 50-150 (at ef 200 a 5-wave cap
@@ -234,7 +235,8 @@ uint32_t overflow_waves(uint64_t nq) {
 
 // Device scratch of such a batch: the overflow list (nq + 2 words) and the
 // overflow kernel's per-wave visited tables in device memory (2^h2_gvis_bits
-// entries each), for queries whose set outgrows half their LDS table (none
+// entries each), for queries whose set outgrows three quarters of their LDS
+// table (none
 // when h2_gvis_bits is 0 or the scratch has no_overflow set: such queries
 // fail); the failure flag and the work-queue head; the GEMM-rotated queries.
 lqg_status h2_reserve(const lqg_index* h, const H2Cfg& cfg, uint64_t nq, Scratch* sc) {

@@ -264,7 +264,7 @@ def test_poll_capacity_below_k_raises_and_keeps_ticket(tmp_path):
 def test_overflow_fails_only_its_own_request(tmp_path):
     """FastScan visited-set overflow is per request.  With no device overflow
     table and a 64-entry LDS table, a query fails once its visited set passes
-    32 nodes; the device API marks exactly those queries (E = 0xFFFFFFFF, ids
+    48 nodes; the device API marks exactly those queries (E = 0xFFFFFFFF, ids
     0xFFFFFFFF), the host batch call fails naming the overflow, and in serving
     only those requests fail while every other request of the same batches
     is handed back with the results it has searched alone.  The kernel exits
