@@ -46,9 +46,14 @@
 namespace lqg {
 
 struct Layout {
-    uint32_t qv, tb, qb, lut, pl, st, hv, wave_bytes;
+    uint32_t qv, tb, qb, lut, pl, st, hv, cnt, wave_bytes;
     int has_qv;
 };
+
+// Pool classes from this capacity on admit a pass's candidates by binary
+// search and a prefix sum (h2_search_one) instead of comparing each candidate
+// with every register slot of the pool.
+constexpr int kRankedPool = 512;
 
 __host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
 
@@ -75,6 +80,9 @@ __host__ __device__ inline Layout h2_layout(uint32_t dim, uint32_t p, uint32_t e
     l.st = o;
     o += prep > stage ? prep : stage;
     l.hv = o;   o += 4u << hash_bits;
+    // ranked admission (pool classes from kRankedPool on): one counter per
+    // pool position
+    l.cnt = o;  o += ef_cap >= static_cast<uint32_t>(kRankedPool) ? 4u * ef_cap : 0u;
     l.wave_bytes = o;
     return l;
 }
@@ -108,6 +116,7 @@ struct H2Lds {
     uint4* st;        // pool merge staging, once the LUT is built (shares tb's memory):
                       // EF_CAP + 1 records {key lo, key hi, id, 0}
     uint32_t* hv;     // visited table (1 << hash_bits)
+    uint32_t* cnt;    // ranked admission: EF_CAP counters, zero between passes
 };
 
 __device__ __forceinline__ H2Lds carve(unsigned char* base, const Layout& l) {
@@ -119,6 +128,7 @@ __device__ __forceinline__ H2Lds carve(unsigned char* base, const Layout& l) {
     w.pl = reinterpret_cast<uint4*>(base + l.pl);
     w.st = reinterpret_cast<uint4*>(base + l.st);
     w.hv = reinterpret_cast<uint32_t*>(base + l.hv);
+    w.cnt = reinterpret_cast<uint32_t*>(base + l.cnt);
     return w;
 }
 
@@ -703,6 +713,15 @@ __device__ bool h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
     // pool has not reached yet must read as empty entries
     for (uint32_t i = lane; i <= static_cast<uint32_t>(EF_CAP); i += kWave)
         w.st[i] = make_uint4(kEmpty, kEmpty, kEmpty, 0u);
+    if constexpr (EF_CAP >= kRankedPool) {
+        // ranked admission searches the staging copy of the pool, so it must
+        // hold the entry point from the start
+        if (lane == 0) {
+            const uint64_t k0 = sb_key(3.402823466e+38f, 0);
+            w.st[0] = make_uint4(static_cast<uint32_t>(k0), static_cast<uint32_t>(k0 >> 32), ix.ep, 0u);
+        }
+        for (uint32_t i = lane; i < static_cast<uint32_t>(EF_CAP); i += kWave) w.cnt[i] = 0u;
+    }
     wave_sync();
 
     // search_pool_.insert(entry_point_, FLT_MAX) (qg.hpp:260); empty entries
@@ -811,25 +830,86 @@ __device__ bool h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
             uint64_t mm = __ballot(cand);
             if (mm == 0) continue;
             const uint32_t m3 = static_cast<uint32_t>(__popcll(mm));
-            // per candidate j: lb_j = pool entries below it; per pool entry:
-            // nlt = candidates above it, so it moves up by m3 - nlt
+            // per candidate j: lb_j = pool entries below it and rank = the
+            // candidates below it; per pool entry i: the candidates below it,
+            // #{j : lb_j <= i}, by which it moves up
             uint32_t lb = 0, rank = 0;
-            uint32_t nlt[S];
+            uint32_t up[S];
+            if constexpr (EF_CAP >= kRankedPool) {
+                // lb by binary search over the staging copy of the pool (as of
+                // the last merge: keys ascending at positions below size; only
+                // checked bits, bit 0, may differ from the registers, and keys
+                // differ above bit 0), then a counter per position
+                if (cand) {
+                    uint32_t pos = 0;
 #pragma unroll
-            for (int s = 0; s < S; ++s) nlt[s] = 0;
-            while (mm) {
-                const int j = static_cast<int>(__builtin_ctzll(mm));
-                mm &= mm - 1;
-                const uint64_t kj = readlane64(key, j);
-                uint32_t below = 0;
-#pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    const bool lt = P[s] < kj;
-                    below += static_cast<uint32_t>(__popcll(__ballot(lt)));
-                    nlt[s] += lt ? 1u : 0u;
+                    for (uint32_t step = static_cast<uint32_t>(EF_CAP); step > 0u; step >>= 1) {
+                        const uint32_t t = pos + step;
+                        if (t <= size) {
+                            const uint2 e = *reinterpret_cast<const uint2*>(&w.st[t - 1u]);
+                            if (((static_cast<uint64_t>(e.y) << 32) | e.x) < key) pos = t;
+                        }
+                    }
+                    lb = pos;  // below ef - 1 < EF_CAP: the pool's bound admits it
+                    atomicAdd(&w.cnt[lb], 1u);
                 }
-                lb = static_cast<int>(lane) == j ? below : lb;
-                rank += kj < key ? 1u : 0u;
+                while (mm) {
+                    const int j = static_cast<int>(__builtin_ctzll(mm));
+                    mm &= mm - 1;
+                    rank += readlane64(key, j) < key ? 1u : 0u;
+                }
+                wave_sync();
+                // inclusive prefix sums of the counters: lane l sums positions
+                // S l .. S l + S - 1 in place, then adds the lanes below
+                constexpr int V = S / 4;
+                uint4* run = reinterpret_cast<uint4*>(w.cnt) + V * lane;
+                uint4 c[V];
+                uint32_t acc = 0;
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    c[v] = run[v];
+                    c[v].x += acc;
+                    c[v].y += c[v].x;
+                    c[v].z += c[v].y;
+                    c[v].w += c[v].z;
+                    acc = c[v].w;
+                }
+                uint32_t incl = acc;
+#pragma unroll
+                for (int d = 1; d < kWave; d <<= 1) {
+                    const uint32_t t = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), d));
+                    if (lane >= static_cast<uint32_t>(d)) incl += t;
+                }
+                const uint32_t off = incl - acc;
+#pragma unroll
+                for (int v = 0; v < V; ++v)
+                    run[v] = make_uint4(c[v].x + off, c[v].y + off, c[v].z + off, c[v].w + off);
+                wave_sync();
+#pragma unroll
+                for (int s = 0; s < S; ++s) up[s] = w.cnt[static_cast<uint32_t>(s * kWave) + lane];
+                wave_sync();
+#pragma unroll
+                for (int v = 0; v < V; ++v) run[v] = make_uint4(0u, 0u, 0u, 0u);
+            } else {
+                uint32_t nlt[S];  // candidates above each pool entry
+#pragma unroll
+                for (int s = 0; s < S; ++s) nlt[s] = 0;
+                while (mm) {
+                    const int j = static_cast<int>(__builtin_ctzll(mm));
+                    mm &= mm - 1;
+                    const uint64_t kj = readlane64(key, j);
+                    uint32_t below = 0;
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        const bool lt = P[s] < kj;
+                        below += static_cast<uint32_t>(__popcll(__ballot(lt)));
+                        nlt[s] += lt ? 1u : 0u;
+                    }
+                    lb = static_cast<int>(lane) == j ? below : lb;
+                    rank += kj < key ? 1u : 0u;
+                }
+#pragma unroll
+                for (int s = 0; s < S; ++s) up[s] = m3 - nlt[s];
             }
             const uint32_t nsize = min(size + m3, ef);
             // merge through LDS: each entry and candidate goes to its new
@@ -838,7 +918,7 @@ __device__ bool h2_search_one(const H2Dev& ix, const H2Job& job, uint32_t q, con
             // empty entries they were given at the start
 #pragma unroll
             for (int s = 0; s < S; ++s) {
-                const uint32_t np = static_cast<uint32_t>(s * kWave) + lane + m3 - nlt[s];
+                const uint32_t np = static_cast<uint32_t>(s * kWave) + lane + up[s];
                 const uint32_t dst = np < nsize ? np : static_cast<uint32_t>(EF_CAP);
                 w.st[dst] = make_uint4(static_cast<uint32_t>(P[s]), static_cast<uint32_t>(P[s] >> 32),
                                        I[s], 0u);
