@@ -347,7 +347,17 @@ struct WaveLds {
     uint64_t* stage;  // pool permutation staging (EF_CAP)
     uint32_t* bids;   // fresh candidate ids (64)
     uint32_t* cache;  // visited cache (1 << cache_bits)
+    uint32_t* cnt;    // ranked admission: EF_CAP counters, zero between passes
 };
+
+// Pool classes from this capacity on admit a pass's candidates by binary
+// search and a prefix sum (search_one) instead of comparing each candidate
+// with every register slot of the pool.  200k x 960 degree 64, 1,000 queries,
+// unsplit kernel: ef U{64..1024} (16 slots) 11.3 -> 10.3 ms; ef U{64..512}
+// (8 slots) 5.42 -> 5.68 ms, so the 512 class keeps the slot compares.  Not
+// in the split kernel, whose workgroup would not fit four to a CU with the
+// counters (split 2 at ef U{64..1024}: 10.7 -> 11.1 ms).
+constexpr int kRankedSq8 = 1024;
 
 // Split evaluation (h1_split_kernel): a query's workgroup of W waves shares
 // each pass's distance evaluation.  Wave 0 walks the query; before each pass it
@@ -487,6 +497,8 @@ __device__ __forceinline__ void search_one(const DevIndex& ix, const SearchJob& 
     }
     const uint32_t csize = 1u << cache_bits;
     for (uint32_t i = lane; i < csize; i += kWave) w.cache[i] = kInvalid;
+    if constexpr (EF_CAP >= kRankedSq8 && W == 1)
+        for (uint32_t i = lane; i < static_cast<uint32_t>(EF_CAP); i += kWave) w.cnt[i] = 0u;
     if (lane == 0) {
         w.bids[0] = ix.ep;
         w.cache[(ix.ep * 2654435761u) >> (32 - cache_bits)] = ix.ep;
@@ -627,54 +639,108 @@ __device__ __forceinline__ void search_one(const DevIndex& ix, const SearchJob& 
             bool cand = key != ~0ull && (size < ef || key < last);
             uint64_t mm = __ballot(cand);
             if (mm == 0) continue;
-            // One pass over the candidates (broadcast from their lanes): each
-            // candidate's position below the pool (lb, written into its lane),
-            // its rank among the candidates, and every pool entry's shift (the
-            // candidates below it).  Candidate keys have bit 0 clear, so these
-            // plain comparisons order by (distance, id) as long as no candidate is
-            // already in the pool -- a node evaluated before whose cache entry
-            // was overwritten.  Such a candidate's pool copy is the entry at its
-            // position lb; the stage array holds the pool as of the last merge
-            // (only expanded bits may differ), so one LDS read per candidate
-            // finds them, and only then is the pass redone with duplicates
-            // dropped.
             uint32_t lb = 0, rank = 0;
             uint32_t shift[S];
+            if constexpr (EF_CAP >= kRankedSq8 && W == 1) {
+                // lb by binary search over the stage array (keys ascending at
+                // positions below size; only expanded bits, bit 0, may differ
+                // from the registers, and no comparison with a candidate,
+                // whose bit 0 is clear, depends on them), then the duplicate
+                // check below, then a counter per position: entry i moves up
+                // by the candidates with lb <= i (their prefix sum)
+                if (cand) {
+                    uint32_t pos = 0;
 #pragma unroll
-            for (int s = 0; s < S; ++s) shift[s] = 0;
-            {
-                uint64_t rest = mm;
-                while (rest) {
-                    const int j = static_cast<int>(__builtin_ctzll(rest));
-                    rest &= rest - 1;
-                    const uint64_t kj = readlane64(key, j);
-                    uint32_t below = 0;
-#pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        const bool lt = P[s] < kj;
-                        below += static_cast<uint32_t>(__popcll(__ballot(lt)));
-                        shift[s] += lt ? 0u : 1u;
+                    for (uint32_t step = static_cast<uint32_t>(EF_CAP); step > 0u; step >>= 1) {
+                        const uint32_t t = pos + step;
+                        if (t <= size && w.stage[t - 1u] < key) pos = t;
                     }
-                    rank += kj < key ? 1u : 0u;
-                    if (static_cast<int>(lane) == j) lb = below;
+                    lb = pos;
                 }
-            }
-            bool drop = cand && lb < size && ((w.stage[lb] ^ key) >> 1) == 0;
-            if (__ballot(drop)) {
-                // slow pass with the duplicates dropped
-                cand = cand && !drop;
+                cand = cand && !(lb < size && ((w.stage[lb] ^ key) >> 1) == 0);
                 mm = __ballot(cand);
-                rank = 0;
+                if (mm == 0) continue;
+                if (cand) atomicAdd(&w.cnt[lb], 1u);  // lb <= ef - 1 < EF_CAP
+                for (uint64_t rest = mm; rest; rest &= rest - 1)
+                    rank += readlane64(key, static_cast<int>(__builtin_ctzll(rest))) < key ? 1u : 0u;
+                wave_sync();
+                constexpr int V = S / 4;  // lane l sums positions S l .. S l + S - 1
+                uint4* run = reinterpret_cast<uint4*>(w.cnt) + V * lane;
+                uint4 c[V];
+                uint32_t acc = 0;
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    c[v] = run[v];
+                    c[v].x += acc;
+                    c[v].y += c[v].x;
+                    c[v].z += c[v].y;
+                    c[v].w += c[v].z;
+                    acc = c[v].w;
+                }
+                uint32_t incl = acc;
+#pragma unroll
+                for (int d = 1; d < kWave; d <<= 1) {
+                    const uint32_t t = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), d));
+                    if (lane >= static_cast<uint32_t>(d)) incl += t;
+                }
+                const uint32_t off = incl - acc;
+#pragma unroll
+                for (int v = 0; v < V; ++v)
+                    run[v] = make_uint4(c[v].x + off, c[v].y + off, c[v].z + off, c[v].w + off);
+                wave_sync();
+#pragma unroll
+                for (int s = 0; s < S; ++s) shift[s] = w.cnt[static_cast<uint32_t>(s * kWave) + lane];
+                wave_sync();
+#pragma unroll
+                for (int v = 0; v < V; ++v) run[v] = make_uint4(0u, 0u, 0u, 0u);
+            } else {
+                // One pass over the candidates (broadcast from their lanes): each
+                // candidate's position below the pool (lb, written into its lane),
+                // its rank among the candidates, and every pool entry's shift (the
+                // candidates below it).  Candidate keys have bit 0 clear, so these
+                // plain comparisons order by (distance, id) as long as no candidate is
+                // already in the pool -- a node evaluated before whose cache entry
+                // was overwritten.  Such a candidate's pool copy is the entry at its
+                // position lb; the stage array holds the pool as of the last merge
+                // (only expanded bits may differ), so one LDS read per candidate
+                // finds them, and only then is the pass redone with duplicates
+                // dropped.
 #pragma unroll
                 for (int s = 0; s < S; ++s) shift[s] = 0;
-                uint64_t rest = mm;
-                while (rest) {
-                    const int j = static_cast<int>(__builtin_ctzll(rest));
-                    rest &= rest - 1;
-                    const uint64_t kj = readlane64(key, j);
-                    rank += kj < key ? 1u : 0u;
+                {
+                    uint64_t rest = mm;
+                    while (rest) {
+                        const int j = static_cast<int>(__builtin_ctzll(rest));
+                        rest &= rest - 1;
+                        const uint64_t kj = readlane64(key, j);
+                        uint32_t below = 0;
 #pragma unroll
-                    for (int s = 0; s < S; ++s) shift[s] += kj < P[s] ? 1u : 0u;
+                        for (int s = 0; s < S; ++s) {
+                            const bool lt = P[s] < kj;
+                            below += static_cast<uint32_t>(__popcll(__ballot(lt)));
+                            shift[s] += lt ? 0u : 1u;
+                        }
+                        rank += kj < key ? 1u : 0u;
+                        if (static_cast<int>(lane) == j) lb = below;
+                    }
+                }
+                bool drop = cand && lb < size && ((w.stage[lb] ^ key) >> 1) == 0;
+                if (__ballot(drop)) {
+                    // slow pass with the duplicates dropped
+                    cand = cand && !drop;
+                    mm = __ballot(cand);
+                    rank = 0;
+#pragma unroll
+                    for (int s = 0; s < S; ++s) shift[s] = 0;
+                    uint64_t rest = mm;
+                    while (rest) {
+                        const int j = static_cast<int>(__builtin_ctzll(rest));
+                        rest &= rest - 1;
+                        const uint64_t kj = readlane64(key, j);
+                        rank += kj < key ? 1u : 0u;
+#pragma unroll
+                        for (int s = 0; s < S; ++s) shift[s] += kj < P[s] ? 1u : 0u;
+                    }
                 }
             }
             const uint32_t m3 = static_cast<uint32_t>(__popcll(__ballot(cand)));
@@ -759,6 +825,7 @@ __device__ __forceinline__ void search_kernel_body(const DevIndex& ix, const Sea
     w.stage = reinterpret_cast<uint64_t*>(wb + 4u * dim_pad);
     w.bids = reinterpret_cast<uint32_t*>(w.stage + EF_CAP);
     w.cache = w.bids + kWave;
+    w.cnt = w.cache + (1u << cache_bits);
 
     // a float launch after an integer launch that left nothing: done
     if (EVAL == EVAL_FLOAT && job.skip_int && job.left && *job.left == 0u) return;
@@ -966,6 +1033,7 @@ __global__ void __launch_bounds__(128) h1_latency_kernel(DevIndex ix, SearchJob 
     w.stage = reinterpret_cast<uint64_t*>(wb + 4u * dim_pad);
     w.bids = reinterpret_cast<uint32_t*>(w.stage + EF_CAP);
     w.cache = w.bids + kWave;
+    w.cnt = w.cache + (1u << cache_bits);
     uint32_t* done = reinterpret_cast<uint32_t*>(wb + wave_bytes);
     // the mirror starts empty (keys ~0 read as expanded)
     for (uint32_t i = threadIdx.x; i < static_cast<uint32_t>(EF_CAP); i += blockDim.x)
@@ -1016,6 +1084,7 @@ h1_split_kernel(DevIndex ix, SearchJob job,
     w.stage = reinterpret_cast<uint64_t*>(wb + 4u * dim_pad);
     w.bids = reinterpret_cast<uint32_t*>(w.stage + EF_CAP);
     w.cache = w.bids + kWave;
+    w.cnt = w.cache + (1u << cache_bits);
     SplitLds* sp = reinterpret_cast<SplitLds*>(wb + wave_bytes);
     __syncthreads();
     // a float launch after an integer launch that left nothing: done (the
@@ -1135,6 +1204,7 @@ __global__ void __launch_bounds__(128) h1_spec_kernel(DevIndex ix, SearchJob job
     w.stage = reinterpret_cast<uint64_t*>(wb + 4u * dim_pad);
     w.bids = reinterpret_cast<uint32_t*>(w.stage + EF_CAP);
     w.cache = w.bids + kWave;
+    w.cnt = w.cache + (1u << cache_bits);
     uint32_t* done = reinterpret_cast<uint32_t*>(wb + wave_bytes);
     SpecLds* sp = reinterpret_cast<SpecLds*>(wb + wave_bytes + 16u);
     for (uint32_t i = threadIdx.x; i < static_cast<uint32_t>(EF_CAP); i += blockDim.x)
@@ -1256,10 +1326,14 @@ KernelFn select_kernel(const LaunchCfg& cfg) {
                                 : select_for_eval<EVAL_FLOAT>(cfg.ef_cap, cfg.occupancy);
 }
 
-void geometry(const DevIndex& ix, const LaunchCfg& cfg, uint32_t& dim_pad, uint32_t& wave_bytes) {
+// with_cnt: the ranked admission's counters (kernels other than the split
+// kernel).
+void geometry(const DevIndex& ix, const LaunchCfg& cfg, uint32_t& dim_pad, uint32_t& wave_bytes,
+              bool with_cnt = true) {
     dim_pad = (ix.dim + 3u) & ~3u;
     size_t wb = 4u * dim_pad + 8u * static_cast<size_t>(cfg.ef_cap) + 4u * kWave +
-                (4u << cfg.cache_bits);
+                (4u << cfg.cache_bits) +
+                (with_cnt && cfg.ef_cap >= kRankedSq8 ? 4u * static_cast<size_t>(cfg.ef_cap) : 0u);
     wave_bytes = static_cast<uint32_t>((wb + 15u) & ~static_cast<size_t>(15u));
 }
 
@@ -1302,7 +1376,7 @@ hipError_t launch_search(const DevIndex& ix, const SearchJob& job, const LaunchC
         KernelFn k = select_split(cfg);
         if (!k || cfg.cache_bits < 4 || cfg.cache_bits > 14) return hipErrorInvalidValue;
         uint32_t dim_pad, wave_bytes;
-        geometry(ix, cfg, dim_pad, wave_bytes);
+        geometry(ix, cfg, dim_pad, wave_bytes, false);
         const size_t lds = 8u * static_cast<size_t>(dim_pad) + wave_bytes + sizeof(SplitLds);
         if (lds > 64 * 1024) {
             hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),

@@ -52,7 +52,8 @@ struct Layout {
 
 // Pool classes from this capacity on admit a pass's candidates by binary
 // search and a prefix sum (h2_search_one) instead of comparing each candidate
-// with every register slot of the pool.
+// with every register slot of the pool.  200k x 960 degree 64, 1,000 queries:
+// ef U{64..1024} 4.57 -> 3.72 ms, ef U{64..512} 2.74 -> 2.49 ms.
 constexpr int kRankedPool = 512;
 
 __host__ __device__ inline uint32_t align16(uint32_t x) { return (x + 15u) & ~15u; }
