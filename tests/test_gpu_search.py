@@ -250,13 +250,14 @@ def test_split_is_default_at_high_dimension(oracle_mod, tmp_path):
 
 def test_autotune_and_prefetch_depth(oracle_mod, tmp_path):
     """The default prefetch depth of small launches survives tuning on a large
-    batch (which times the batch kernel only); tuning on a small set picks a
-    depth among the latency-mode ones or the batch kernel; results stay the
+    batch (above the 1,024 queries of a latency-mode launch at D <= 256, so
+    it times the batch kernel only); tuning on a small set picks a depth
+    among the latency-mode ones or the batch kernel; results stay the
     oracle's throughout."""
     arr, base = make_random_index(3000, 64, 32, seed=8, knn=True)
     ix = device_from_arrays(arr, tmp_path)
     rng = np.random.default_rng(9)
-    Q = (base[rng.integers(0, 3000, 400)] + rng.normal(0, 0.05, (400, 64))).astype(np.float32)
+    Q = (base[rng.integers(0, 3000, 1100)] + rng.normal(0, 0.05, (1100, 64))).astype(np.float32)
     assert ix.get_launch(50)["prefetch"] == 2
     big = ix.autotune(Q, 50, 10)
     assert big["prefetch"] == 2 and ix.get_launch(50)["prefetch"] == 2
