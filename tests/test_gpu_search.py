@@ -354,6 +354,31 @@ def test_launch_settings_do_not_change_results(oracle_mod, tmp_path, occ, bits, 
         assert st["expansions"] == int(est[:, 0].sum())
 
 
+@pytest.mark.parametrize("bits,prefetch", [(4, 0), (12, 0), (4, 2)])
+def test_large_pool_ranked_admission(oracle_mod, tmp_path, bits, prefetch):
+    """Pool class 1024 admits a pass's candidates by binary search over the
+    pool's staging copy and a prefix sum of per-position counters.  With a
+    16-entry visited cache the kernel re-evaluates many nodes (more
+    evaluations than the oracle's distinct ones), and those still in the pool
+    reach admission again (the duplicate drop); ids, distance bits and
+    expansion counts equal the oracle's at ef 600 and 1024 in both orders, in
+    the batch kernel and in latency mode."""
+    arr, base = make_random_index(6000, 64, 32, seed=31, knn=True)
+    ix = device_from_arrays(arr, tmp_path)
+    rng = np.random.default_rng(6)
+    Q = (base[rng.integers(0, 6000, 120)] + rng.normal(0, 0.05, size=(120, 64))).astype(np.float32)
+    ix.set_launch(0, waves_per_group=2, cache_bits=bits, occupancy=0, prefetch=prefetch)
+    for ef in (600, 1024):
+        for order in ("nearest", "reference"):
+            ids, dis, st = ix.search_batch(Q, 10, ef=ef, order=order, return_distances=True,
+                                           return_stats=True)
+            exp, exp_d, est = oracle_batch(oracle_mod, arr, Q, ef, 10, order)
+            assert_same(ids, dis, exp, exp_d)
+            assert st["expansions"] == int(est[:, 0].sum())
+            if bits == 4:
+                assert st["evaluations"] > int(est[:, 1].sum())
+
+
 def test_autotune_keeps_results(oracle_mod, tmp_path):
     arr, base = make_random_index(8000, 128, 32, seed=12, kind="int", knn=True)
     ix = device_from_arrays(arr, tmp_path)
