@@ -347,16 +347,18 @@ struct WaveLds {
     uint64_t* stage;  // pool permutation staging (EF_CAP)
     uint32_t* bids;   // fresh candidate ids (64)
     uint32_t* cache;  // visited cache (1 << cache_bits)
-    uint32_t* cnt;    // ranked admission: EF_CAP counters, zero between passes
+    uint32_t* cnt;    // ranked admission: EF_CAP 16-bit counters (two per word),
+                      // zero between passes
 };
 
 // Pool classes from this capacity on admit a pass's candidates by binary
 // search and a prefix sum (search_one) instead of comparing each candidate
 // with every register slot of the pool.  200k x 960 degree 64, 1,000 queries,
 // unsplit kernel: ef U{64..1024} (16 slots) 11.3 -> 10.3 ms; ef U{64..512}
-// (8 slots) 5.42 -> 5.68 ms, so the 512 class keeps the slot compares.  Not
-// in the split kernel, whose workgroup would not fit four to a CU with the
-// counters (split 2 at ef U{64..1024}: 10.7 -> 11.1 ms).
+// (8 slots) 5.42 -> 5.68 ms, so the 512 class keeps the slot compares.  The
+// counters are 16-bit (a pass admits at most 64 candidates): with 32-bit ones
+// the split kernel's 960-d workgroup needed 64 bytes more than a quarter of
+// the LDS and ran three to a CU (split 2: 10.7 -> 11.1 ms).
 constexpr int kRankedSq8 = 1024;
 
 // Split evaluation (h1_split_kernel): a query's workgroup of W waves shares
@@ -497,8 +499,8 @@ __device__ __forceinline__ void search_one(const DevIndex& ix, const SearchJob& 
     }
     const uint32_t csize = 1u << cache_bits;
     for (uint32_t i = lane; i < csize; i += kWave) w.cache[i] = kInvalid;
-    if constexpr (EF_CAP >= kRankedSq8 && W == 1)
-        for (uint32_t i = lane; i < static_cast<uint32_t>(EF_CAP); i += kWave) w.cnt[i] = 0u;
+    if constexpr (EF_CAP >= kRankedSq8)
+        for (uint32_t i = lane; i < static_cast<uint32_t>(EF_CAP) / 2u; i += kWave) w.cnt[i] = 0u;
     if (lane == 0) {
         w.bids[0] = ix.ep;
         w.cache[(ix.ep * 2654435761u) >> (32 - cache_bits)] = ix.ep;
@@ -641,7 +643,7 @@ __device__ __forceinline__ void search_one(const DevIndex& ix, const SearchJob& 
             if (mm == 0) continue;
             uint32_t lb = 0, rank = 0;
             uint32_t shift[S];
-            if constexpr (EF_CAP >= kRankedSq8 && W == 1) {
+            if constexpr (EF_CAP >= kRankedSq8) {
                 // lb by binary search over the stage array (keys ascending at
                 // positions below size; only expanded bits, bit 0, may differ
                 // from the registers, and no comparison with a candidate,
@@ -660,22 +662,26 @@ __device__ __forceinline__ void search_one(const DevIndex& ix, const SearchJob& 
                 cand = cand && !(lb < size && ((w.stage[lb] ^ key) >> 1) == 0);
                 mm = __ballot(cand);
                 if (mm == 0) continue;
-                if (cand) atomicAdd(&w.cnt[lb], 1u);  // lb <= ef - 1 < EF_CAP
+                // 16-bit counters, position p in half p & 1 of word p >> 1;
+                // no half passes 64, so a word-wide add never carries
+                if (cand) atomicAdd(&w.cnt[lb >> 1], 1u << (16u * (lb & 1u)));  // lb < EF_CAP
                 for (uint64_t rest = mm; rest; rest &= rest - 1)
                     rank += readlane64(key, static_cast<int>(__builtin_ctzll(rest))) < key ? 1u : 0u;
                 wave_sync();
-                constexpr int V = S / 4;  // lane l sums positions S l .. S l + S - 1
+                constexpr int V = S / 8;  // lane l sums positions S l .. S l + S - 1
                 uint4* run = reinterpret_cast<uint4*>(w.cnt) + V * lane;
-                uint4 c[V];
+                uint32_t c[4 * V];
                 uint32_t acc = 0;
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
-                    c[v] = run[v];
-                    c[v].x += acc;
-                    c[v].y += c[v].x;
-                    c[v].z += c[v].y;
-                    c[v].w += c[v].z;
-                    acc = c[v].w;
+                    const uint4 x = run[v];
+                    const uint32_t xs[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t p0 = acc + (xs[u] & 0xFFFFu);
+                        acc = p0 + (xs[u] >> 16);
+                        c[4 * v + u] = p0 | (acc << 16);
+                    }
                 }
                 uint32_t incl = acc;
 #pragma unroll
@@ -683,13 +689,15 @@ __device__ __forceinline__ void search_one(const DevIndex& ix, const SearchJob& 
                     const uint32_t t = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), d));
                     if (lane >= static_cast<uint32_t>(d)) incl += t;
                 }
-                const uint32_t off = incl - acc;
+                const uint32_t off = (incl - acc) * 0x10001u;  // added to both halves
 #pragma unroll
                 for (int v = 0; v < V; ++v)
-                    run[v] = make_uint4(c[v].x + off, c[v].y + off, c[v].z + off, c[v].w + off);
+                    run[v] = make_uint4(c[4 * v] + off, c[4 * v + 1] + off, c[4 * v + 2] + off,
+                                        c[4 * v + 3] + off);
                 wave_sync();
+                const uint16_t* c16 = reinterpret_cast<const uint16_t*>(w.cnt);
 #pragma unroll
-                for (int s = 0; s < S; ++s) shift[s] = w.cnt[static_cast<uint32_t>(s * kWave) + lane];
+                for (int s = 0; s < S; ++s) shift[s] = c16[static_cast<uint32_t>(s * kWave) + lane];
                 wave_sync();
 #pragma unroll
                 for (int v = 0; v < V; ++v) run[v] = make_uint4(0u, 0u, 0u, 0u);
@@ -1326,14 +1334,11 @@ KernelFn select_kernel(const LaunchCfg& cfg) {
                                 : select_for_eval<EVAL_FLOAT>(cfg.ef_cap, cfg.occupancy);
 }
 
-// with_cnt: the ranked admission's counters (kernels other than the split
-// kernel).
-void geometry(const DevIndex& ix, const LaunchCfg& cfg, uint32_t& dim_pad, uint32_t& wave_bytes,
-              bool with_cnt = true) {
+void geometry(const DevIndex& ix, const LaunchCfg& cfg, uint32_t& dim_pad, uint32_t& wave_bytes) {
     dim_pad = (ix.dim + 3u) & ~3u;
     size_t wb = 4u * dim_pad + 8u * static_cast<size_t>(cfg.ef_cap) + 4u * kWave +
                 (4u << cfg.cache_bits) +
-                (with_cnt && cfg.ef_cap >= kRankedSq8 ? 4u * static_cast<size_t>(cfg.ef_cap) : 0u);
+                (cfg.ef_cap >= kRankedSq8 ? 2u * static_cast<size_t>(cfg.ef_cap) : 0u);
     wave_bytes = static_cast<uint32_t>((wb + 15u) & ~static_cast<size_t>(15u));
 }
 
@@ -1376,7 +1381,7 @@ hipError_t launch_search(const DevIndex& ix, const SearchJob& job, const LaunchC
         KernelFn k = select_split(cfg);
         if (!k || cfg.cache_bits < 4 || cfg.cache_bits > 14) return hipErrorInvalidValue;
         uint32_t dim_pad, wave_bytes;
-        geometry(ix, cfg, dim_pad, wave_bytes, false);
+        geometry(ix, cfg, dim_pad, wave_bytes);
         const size_t lds = 8u * static_cast<size_t>(dim_pad) + wave_bytes + sizeof(SplitLds);
         if (lds > 64 * 1024) {
             hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
