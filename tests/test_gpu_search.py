@@ -159,7 +159,7 @@ def test_latency_mode_matches_oracle(oracle_mod, tmp_path, prefetch, kind, d, de
 def test_split_evaluation_matches_oracle(oracle_mod, tmp_path, split, kind, d, deg):
     """Split evaluation (one query per workgroup of 2 or 4 waves sharing each
     pass's distance evaluation): ids and distance bits equal the oracle's in
-    both orders at ef 1, 50, 150 and 600, with per-query ef and k in one
+    both orders at ef 1, 50, 150, 300 and 600, with per-query ef and k in one
     launch, and for the plain search.  The integer index runs it with its
     integer kernel disabled; D = 100 takes the generic row loop."""
     n = 3000 if d <= 128 else 1500
@@ -176,7 +176,7 @@ def test_split_evaluation_matches_oracle(oracle_mod, tmp_path, split, kind, d, d
     if kind == "int":
         Q = np.clip(np.rint(Q), 0, 255).astype(np.float32)
     for order in ("reference", "nearest"):
-        for ef in (1, 50, 150, 600):
+        for ef in (1, 50, 150, 300, 600):
             for m in (1, 37, nq):
                 ids, dis = ix.search_batch(Q[:m], 10, ef=ef, order=order, return_distances=True)
                 exp, exp_d, _ = oracle_batch(oracle_mod, arr, Q[:m], ef, 10, order)
