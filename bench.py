@@ -447,6 +447,47 @@ def wide_leg(args, host):
     return out
 
 
+def wide_probe(args):
+    """--pmc-probe on the 960-d workload of wide_leg (pos_mix --wide-n x 960,
+    degree 64, 1,000 queries at per-query ef U{64..1024} and k {10, 100}, the
+    same draws): --probe-launches launches of the probe's mode (sq8-wide or
+    fastscan-wide) with the --launch settings, then a 1 GiB device copy."""
+    import torch
+
+    from liteqg_amd import datasets
+    from liteqg_amd.symphonyqg import Index
+
+    n, d, deg, nq = args.wide_n, 960, 64, 1000
+    base, Q = datasets.make("pos_mix", n, d, nq, seed=args.seed, centres=512, sigma=0.05)
+    ix = Index("QG", "L2", n, d, deg)
+    ix.build_index(base, 500, num_iter=3)
+    rng = np.random.default_rng(args.seed + 4)
+    ef = rng.integers(64, 1025, nq).astype(np.uint32)
+    k = np.where(rng.random(nq) < 0.5, 10, 100).astype(np.uint32)
+    db = DeviceBatch(ix, Q, int(k.max()), ef_per_query=ef, k_per_query=k)
+    if args.probe_mode == "fastscan-wide":
+        ix.build_sidecar(seed=args.seed)
+        ix.set_mode("fastscan")
+    ef_max = int(ef.max())
+    if args.launch:
+        ix.set_launch(ef_max, *[int(x) for x in args.launch.split(",")])
+    else:
+        ix.autotune(Q, ef_max, int(k.max()))
+    ms, _ = time_launches(db, ef_max, args.probe_launches, 2)
+    db.launch(ef_max, with_stats=True)
+    torch.cuda.synchronize()
+    n_exp, _ = db.counters()
+    log(f"wide probe counters: {int(n_exp.sum())} expansions, {n_exp.mean():.2f} per query, "
+        f"max {int(n_exp.max())}")
+    x = torch.zeros(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"pmc probe: {args.probe_launches} {args.probe_mode} launches of {nq} queries "
+        f"(N={n}), launch {ix.get_launch(ef_max)}, {ms:.4f} ms per launch (HIP events), "
+        f"then a 1 GiB copy")
+
+
 def configs_leg(args, host):
     """Two more workloads at full size, SQ8 path, order=nearest, graphs built on
     the device (3 iterations), FP64 ground truth, 10 timed launches per ef,
@@ -871,19 +912,25 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 runs: build, --probe-launches launches at --ef (default 50) with "
                          "the tuned launch settings, then a 1 GiB device copy (counter scale)")
-    ap.add_argument("--probe-mode", choices=("sq8", "fastscan"), default="sq8",
-                    help="search path of the --pmc-probe launches (fastscan: default ef 200)")
+    ap.add_argument("--probe-mode", choices=("sq8", "fastscan", "sq8-wide", "fastscan-wide"),
+                    default="sq8",
+                    help="search path of the --pmc-probe launches (fastscan: default ef 200; "
+                         "*-wide: the 960-d workload at --wide-n nodes, per-query ef and k)")
     ap.add_argument("--probe-launches", type=int, default=3,
                     help="number of --pmc-probe search launches")
     ap.add_argument("--launch", default="",
                     help="pin the launch settings of the probe instead of autotuning: "
-                         "waves_per_group,cache_bits,occupancy[,groups]")
+                         "waves_per_group,cache_bits,occupancy[,groups[,prefetch[,split]]]")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the query path is data-parallel over queries and "
               "this build runs on a single MI355X; --gpus above 1 is not supported.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.pmc_probe and args.probe_mode.endswith("-wide"):
+        wide_probe(args)
+        return
 
     import torch
 
